@@ -1,0 +1,422 @@
+// Host-side level-3 algorithms: implementation selection, GEMM dispatch, the
+// recursive blocked STRSM, mixed-precision refinement and the adjoints.
+#include "blas3.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+namespace icl {
+
+// ---------------------------------------------------------------------------
+// Registry
+// ---------------------------------------------------------------------------
+const std::vector<const icl_gemm_variant*>& gemm_registry()
+{
+    static const std::vector<const icl_gemm_variant*> all = [] {
+        std::vector<const icl_gemm_variant*> v;
+        int n = 0;
+        const icl_gemm_variant* t = icl_sgemm_f32_variants(&n);
+        for (int i = 0; i < n; ++i) v.push_back(t + i);
+        t = icl_sgemm_bf16x3_variants(&n);
+        for (int i = 0; i < n; ++i) v.push_back(t + i);
+        t = icl_cgemm_f32_variants(&n);
+        for (int i = 0; i < n; ++i) v.push_back(t + i);
+        t = icl_cgemm_bf16x3_variants(&n);
+        for (int i = 0; i < n; ++i) v.push_back(t + i);
+        return v;
+    }();
+    return all;
+}
+
+bool gemm_accept(const icl_gemm_variant& v, const GemmProblem& p) { return v.kind == p.kind && v.math == p.math; }
+
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+bool gemm_needs_edge(const icl_gemm_variant& v, const GemmProblem& p)
+{
+    const icl_gemm_args& g = *p.args;
+    // 16-byte loads along each operand's contiguous dimension need aligned columns.
+    const long vec = p.kind == ICL_KIND_CGEMM ? 2 : 4;
+    const bool even = g.M % v.bm == 0 && g.N % v.bn == 0 && g.K % v.bk == 0;
+    const bool aligned = aligned16(g.A) && aligned16(g.B) && g.lda % vec == 0 && g.ldb % vec == 0;
+    return !(even && aligned);
+}
+
+// Score model: the fraction of the last dispatch wave that is filled, times the
+// tile's relative MFMA efficiency (bigger wave tiles re-use each LDS fragment more),
+// with a small bonus for problems that can use the unchecked instantiation.
+float gemm_score(const icl_gemm_variant& v, const GemmProblem& p)
+{
+    const icl_gemm_args& g = *p.args;
+    constexpr double kCUs = 256.0;
+    const double tiles = double((g.M + v.bm - 1) / v.bm) * double((g.N + v.bn - 1) / v.bn);
+    const double waves = tiles / kCUs;
+    const double fill = waves >= 1.0 ? waves / std::ceil(waves) : waves;
+    const double tile_eff = double(v.bm * v.bn) / (256.0 * 256.0);
+    const double eff = 0.75 + 0.25 * tile_eff;
+    // useful work fraction of the padded tiles
+    const double useful = double(g.M) * double(g.N) / (tiles * v.bm * v.bn);
+    double s = fill * eff * useful;
+    if (!gemm_needs_edge(v, p)) s *= 1.05;
+    return (float)s;
+}
+
+static bool forced_is_gemm_of_kind(const std::string& name, int kind)
+{
+    for (const icl_gemm_variant* v : gemm_registry())
+        if (name == v->name) return v->kind == kind;
+    return false;
+}
+
+const icl_gemm_variant* gemm_select(iclblasContext* h, const GemmProblem& p)
+{
+    if (!h->forced_impl.empty() && forced_is_gemm_of_kind(h->forced_impl, p.kind)) {
+        for (const icl_gemm_variant* v : gemm_registry())
+            if (h->forced_impl == v->name) return v;
+    }
+    const icl_gemm_variant* best = nullptr;
+    float best_score = -1.0f;
+    for (const icl_gemm_variant* v : gemm_registry()) {
+        if (!gemm_accept(*v, p)) continue;
+        const float s = gemm_score(*v, p);
+        if (s > best_score) {
+            best = v;
+            best_score = s;
+        }
+    }
+    if (!best) throw status_error(ICLBLAS_STATUS_NOT_SUPPORTED, "no GEMM implementation accepts the parameters");
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// GEMM
+// ---------------------------------------------------------------------------
+static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, const icl_gemm_args& g, bool record)
+{
+    GemmProblem p{kind, math, opa, opb, &g};
+    const icl_gemm_variant* v = gemm_select(h, p);
+    const bool edge = gemm_needs_edge(*v, p);
+    hip_check(v->launch(&g, opa, opb, edge ? 1 : 0, h->stream), v->name);
+    if (record) h->last_impl = v->name;
+}
+
+void sgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int k, float alpha, const float* A,
+               long lda, const float* B, long ldb, float beta, float* C, long ldc, int mask, bool record)
+{
+    if (m <= 0 || n <= 0) return;
+    if (k == 0 || alpha == 0.0f) {
+        // C = beta * C; A and B are not read.
+        hip_check(icl_axpby_2d(m, n, 0.0f, nullptr, 0, beta, C, ldc, h->stream), "axpby");
+        if (mask != ICL_MASK_NONE)
+            throw status_error(ICLBLAS_STATUS_INTERNAL_ERROR, "masked GEMM with k == 0 or alpha == 0");
+        if (record) h->last_impl = "scale_c";
+        return;
+    }
+    icl_gemm_args g{};
+    g.M = m;
+    g.N = n;
+    g.K = k;
+    g.A = A;
+    g.B = B;
+    g.C = C;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.ldc = ldc;
+    g.alpha_re = alpha;
+    g.beta_re = beta;
+    g.mask = mask;
+    run_gemm(h, ICL_KIND_SGEMM, math, opa, opb, g, record);
+}
+
+void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int k, const float alpha[2],
+               const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc)
+{
+    if (m <= 0 || n <= 0) return;
+    if (k == 0 || (alpha[0] == 0.0f && alpha[1] == 0.0f)) {
+        hip_check(icl_caxpby_2d(m, n, 0.0f, 0.0f, nullptr, 0, beta[0], beta[1], C, ldc, h->stream), "caxpby");
+        h->last_impl = "scale_c";
+        return;
+    }
+    icl_gemm_args g{};
+    g.M = m;
+    g.N = n;
+    g.K = k;
+    g.A = A;
+    g.B = B;
+    g.C = C;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.ldc = ldc;
+    g.alpha_re = alpha[0];
+    g.alpha_im = alpha[1];
+    g.beta_re = beta[0];
+    g.beta_im = beta[1];
+    run_gemm(h, ICL_KIND_CGEMM, math, opa, opb, g, true);
+}
+
+// ---------------------------------------------------------------------------
+// STRSM: recursive blocking. Each split solves one half, sends the coupling
+// block through the FP32 GEMM (alpha = -1, beta = 1) and solves the other half;
+// leaves of at most 64 unknowns go to the one-wave-per-64-RHS diagonal solver.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kLeaf = 64;
+
+struct Trsm {
+    iclblasContext* h;
+    bool left, trans, unit, fwd;
+    int m, n;
+    const float* A;
+    long lda;
+    float* B;
+    long ldb;
+    long si, sk; // op(A)(p, q) = A[p * si + q * sk]
+
+    // op(A)[i0.., j0..] as a GEMM operand: pointer and op flag.
+    const float* sub(int i0, int j0, int* op) const
+    {
+        *op = trans ? 1 : 0;
+        return A + (long)i0 * si + (long)j0 * sk;
+    }
+
+    void leaf(int p0, int nb, float s_post) const
+    {
+        icl_trsm_diag_args a{};
+        a.nb = nb;
+        a.unit = unit ? 1 : 0;
+        a.s_pre = 1.0f;
+        a.s_post = s_post;
+        if (left) {
+            a.nlines = n;
+            a.x_sj = ldb;
+            if (fwd) {
+                a.L = A + (long)p0 * (si + sk);
+                a.l_si = si;
+                a.l_sk = sk;
+                a.X = B + p0;
+                a.x_si = 1;
+            } else {
+                const long e = p0 + nb - 1;
+                a.L = A + e * (si + sk);
+                a.l_si = -si;
+                a.l_sk = -sk;
+                a.X = B + e;
+                a.x_si = -1;
+            }
+        } else {
+            a.nlines = m;
+            a.x_sj = 1;
+            if (fwd) {
+                a.L = A + (long)p0 * (si + sk);
+                a.l_si = sk;
+                a.l_sk = si;
+                a.X = B + (long)p0 * ldb;
+                a.x_si = ldb;
+            } else {
+                const long e = p0 + nb - 1;
+                a.L = A + e * (si + sk);
+                a.l_si = -sk;
+                a.l_sk = -si;
+                a.X = B + e * ldb;
+                a.x_si = -ldb;
+            }
+        }
+        hip_check(icl_trsm_diag(&a, h->stream), "trsm_diag");
+    }
+
+    // Solve unknowns [p0, p1) (rows of X for left, columns for right).
+    void solve(int p0, int p1, float s_post) const
+    {
+        const int len = p1 - p0;
+        if (len <= kLeaf) {
+            leaf(p0, len, s_post);
+            return;
+        }
+        const int half = ((len / 2 + kLeaf - 1) / kLeaf) * kLeaf;
+        const int mid = p0 + half;
+        int op = 0;
+        if (left) {
+            if (fwd) {
+                solve(p0, mid, s_post);
+                const float* Asub = sub(mid, p0, &op);
+                sgemm_dev(h, ICL_MATH_FP32, op, 0, p1 - mid, n, mid - p0, -1.0f, Asub, lda, B + p0, ldb, 1.0f,
+                          B + mid, ldb, ICL_MASK_NONE, false);
+                solve(mid, p1, s_post);
+            } else {
+                solve(mid, p1, s_post);
+                const float* Asub = sub(p0, mid, &op);
+                sgemm_dev(h, ICL_MATH_FP32, op, 0, mid - p0, n, p1 - mid, -1.0f, Asub, lda, B + mid, ldb, 1.0f,
+                          B + p0, ldb, ICL_MASK_NONE, false);
+                solve(p0, mid, s_post);
+            }
+        } else {
+            if (fwd) {
+                solve(p0, mid, s_post);
+                const float* Asub = sub(p0, mid, &op);
+                sgemm_dev(h, ICL_MATH_FP32, 0, op, m, p1 - mid, mid - p0, -1.0f, B + (long)p0 * ldb, ldb, Asub, lda,
+                          1.0f, B + (long)mid * ldb, ldb, ICL_MASK_NONE, false);
+                solve(mid, p1, s_post);
+            } else {
+                solve(mid, p1, s_post);
+                const float* Asub = sub(mid, p0, &op);
+                sgemm_dev(h, ICL_MATH_FP32, 0, op, m, mid - p0, p1 - mid, -1.0f, B + (long)mid * ldb, ldb, Asub, lda,
+                          1.0f, B + (long)p0 * ldb, ldb, ICL_MASK_NONE, false);
+                solve(p0, mid, s_post);
+            }
+        }
+    }
+};
+
+} // namespace
+
+void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
+               const float* A, long lda, float* B, long ldb, bool record)
+{
+    if (m <= 0 || n <= 0) return;
+    if (alpha == 0.0f) {
+        // The API contract (reference iclBLAS.h:1713): with alpha == 0 neither A
+        // nor B has to be initialised; X = 0.
+        hip_check(icl_axpby_2d(m, n, 0.0f, nullptr, 0, 0.0f, B, ldb, h->stream), "axpby");
+        if (record) h->last_impl = "strsm_zero";
+        return;
+    }
+    Trsm t{};
+    t.h = h;
+    t.left = side == ICLBLAS_SIDE_LEFT;
+    t.trans = trans != ICLBLAS_OP_N;
+    t.unit = diag == ICLBLAS_DIAG_UNIT;
+    t.m = m;
+    t.n = n;
+    t.A = A;
+    t.lda = lda;
+    t.B = B;
+    t.ldb = ldb;
+    t.si = t.trans ? lda : 1;
+    t.sk = t.trans ? 1 : lda;
+    const bool lower_eff = (uplo == ICLBLAS_FILL_MODE_LOWER) != t.trans; // op(A) lower triangular
+    t.fwd = t.left ? lower_eff : !lower_eff;
+    const int order = t.left ? m : n;
+    if (order <= kLeaf) {
+        // Single diagonal block: alpha applied to the solution, as the reference
+        // kernel does (Strsm_naive.cl:35).
+        t.solve(0, order, alpha);
+    } else {
+        t.solve(0, order, 1.0f);
+        if (alpha != 1.0f) hip_check(icl_axpby_2d(m, n, alpha, B, ldb, 0.0f, B, ldb, h->stream), "axpby");
+    }
+    if (record) h->last_impl = "strsm_blocked_rec64";
+}
+
+void strsm_refined_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
+                       const float* A, long lda, float* B, long ldb, int sweeps, float* B0, float* R)
+{
+    if (m <= 0 || n <= 0) return;
+    if (alpha == 0.0f || sweeps <= 0) {
+        strsm_dev(h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb);
+        return;
+    }
+    // Keep the right-hand side: B0 = B (dense, ld = m).
+    hip_check(hipMemcpy2DAsync(B0, (size_t)m * sizeof(float), B, (size_t)ldb * sizeof(float), (size_t)m * sizeof(float),
+                               (size_t)n, hipMemcpyDeviceToDevice, h->stream),
+              "hipMemcpy2DAsync");
+    strsm_dev(h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, false);
+
+    const bool left = side == ICLBLAS_SIDE_LEFT;
+    const bool tr = trans != ICLBLAS_OP_N;
+    const bool lower_eff = (uplo == ICLBLAS_FILL_MODE_LOWER) != tr;
+    for (int s = 0; s < sweeps; ++s) {
+        icl_residual_args r{};
+        r.M = m;
+        r.N = n;
+        r.K = left ? m : n;
+        if (left) { // R = alpha*B0 - op(A) * X
+            r.P = A;
+            r.ldp = lda;
+            r.p_trans = tr;
+            r.Q = B;
+            r.ldq = ldb;
+            r.q_trans = 0;
+            r.tri_on = 1;
+        } else { // R = alpha*B0 - X * op(A)
+            r.P = B;
+            r.ldp = ldb;
+            r.p_trans = 0;
+            r.Q = A;
+            r.ldq = lda;
+            r.q_trans = tr;
+            r.tri_on = 2;
+        }
+        r.tri_lower = lower_eff ? 1 : 0;
+        r.tri_unit = diag == ICLBLAS_DIAG_UNIT ? 1 : 0;
+        r.B0 = B0;
+        r.ldb0 = m;
+        r.R = R;
+        r.ldr = m;
+        r.alpha = (double)alpha;
+        hip_check(icl_residual_f64(&r, h->stream), "residual_f64");
+        // D = op(A)^-1 R (in place in R), X += D
+        strsm_dev(h, side, uplo, trans, diag, m, n, 1.0f, A, lda, R, m, false);
+        hip_check(icl_axpby_2d(m, n, 1.0f, R, m, 1.0f, B, ldb, h->stream), "axpby");
+    }
+    h->last_impl = "strsm_refined_f64res";
+}
+
+// ---------------------------------------------------------------------------
+// Adjoints
+// ---------------------------------------------------------------------------
+void sgemm_backward_dev(iclblasContext* h, int transa, int transb, int m, int n, int k, float alpha, const float* A,
+                        long lda, const float* B, long ldb, const float* dC, long lddc, float* dA, long ldda,
+                        float* dB, long lddb)
+{
+    const int ta = transa != ICLBLAS_OP_N, tb = transb != ICLBLAS_OP_N;
+    if (dA) {
+        if (!ta) // dA (m x k) = alpha * dC * op(B)^T
+            sgemm_dev(h, ICL_MATH_FP32, 0, tb ? 0 : 1, m, k, n, alpha, dC, lddc, B, ldb, 0.0f, dA, ldda);
+        else // dA (k x m) = alpha * op(B) * dC^T
+            sgemm_dev(h, ICL_MATH_FP32, tb, 1, k, m, n, alpha, B, ldb, dC, lddc, 0.0f, dA, ldda);
+    }
+    if (dB) {
+        if (!tb) // dB (k x n) = alpha * op(A)^T * dC
+            sgemm_dev(h, ICL_MATH_FP32, ta ? 0 : 1, 0, k, n, m, alpha, A, lda, dC, lddc, 0.0f, dB, lddb);
+        else // dB (n x k) = alpha * dC^T * op(A)
+            sgemm_dev(h, ICL_MATH_FP32, 1, ta, n, k, m, alpha, dC, lddc, A, lda, 0.0f, dB, lddb);
+    }
+    h->last_impl = "sgemm_backward";
+}
+
+void strsm_backward_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
+                        const float* A, long lda, const float* X, long ldx, const float* dX, long lddx, float* dA,
+                        long ldda, float* dB, long lddb, float* G)
+{
+    const bool left = side == ICLBLAS_SIDE_LEFT;
+    const int ta = trans != ICLBLAS_OP_N;
+    const int order = left ? m : n;
+    // G = op(A)^-T dX (left) or dX op(A)^-T (right): the solve with op(A)^T.
+    hip_check(hipMemcpy2DAsync(G, (size_t)m * sizeof(float), dX, (size_t)lddx * sizeof(float), (size_t)m * sizeof(float),
+                               (size_t)n, hipMemcpyDeviceToDevice, h->stream),
+              "hipMemcpy2DAsync");
+    strsm_dev(h, side, uplo, ta ? ICLBLAS_OP_N : ICLBLAS_OP_T, diag, m, n, 1.0f, A, lda, G, m, false);
+    if (dB) hip_check(icl_axpby_2d(m, n, alpha, G, m, 0.0f, dB, lddb, h->stream), "axpby");
+    if (dA) {
+        int mask;
+        if (uplo == ICLBLAS_FILL_MODE_LOWER) mask = diag == ICLBLAS_DIAG_UNIT ? ICL_MASK_LOWER_STRICT : ICL_MASK_LOWER;
+        else mask = diag == ICLBLAS_DIAG_UNIT ? ICL_MASK_UPPER_STRICT : ICL_MASK_UPPER;
+        if (left) {
+            if (!ta) // dA = -G * X^T  (m x m, K = n)
+                sgemm_dev(h, ICL_MATH_FP32, 0, 1, order, order, n, -1.0f, G, m, X, ldx, 0.0f, dA, ldda, mask);
+            else // dA = -X * G^T
+                sgemm_dev(h, ICL_MATH_FP32, 0, 1, order, order, n, -1.0f, X, ldx, G, m, 0.0f, dA, ldda, mask);
+        } else {
+            if (!ta) // dA = -X^T * G  (n x n, K = m)
+                sgemm_dev(h, ICL_MATH_FP32, 1, 0, order, order, m, -1.0f, X, ldx, G, m, 0.0f, dA, ldda, mask);
+            else // dA = -G^T * X
+                sgemm_dev(h, ICL_MATH_FP32, 1, 0, order, order, m, -1.0f, G, m, X, ldx, 0.0f, dA, ldda, mask);
+        }
+    }
+    h->last_impl = "strsm_backward";
+}
+
+} // namespace icl

@@ -1,0 +1,62 @@
+// Device-level level-3 algorithms of the host layer. All pointers here are device
+// pointers; work is queued on the handle's stream; errors are thrown as
+// icl::status_error and turned into iclblasStatus_t at the C boundary.
+#pragma once
+#include "handle.hpp"
+#include "kernel_abi.h"
+
+#include <vector>
+
+namespace icl {
+
+// ---- implementation registry (reference: include/iclgpu/functions_base.hpp:149-223,
+//      include/iclgpu/dispatcher.hpp:49-61) ----
+struct GemmProblem {
+    int kind; // ICL_KIND_*
+    int math; // ICL_MATH_*
+    int opa, opb;
+    const icl_gemm_args* args;
+};
+
+// Every registered GEMM implementation, in registration order.
+const std::vector<const icl_gemm_variant*>& gemm_registry();
+
+// accept(): the implementation handles this kind and math mode.
+bool gemm_accept(const icl_gemm_variant& v, const GemmProblem& p);
+// score(): higher is better; see the definition for the model.
+float gemm_score(const icl_gemm_variant& v, const GemmProblem& p);
+// Whether the problem needs the bounds-checked instantiation of `v`.
+bool gemm_needs_edge(const icl_gemm_variant& v, const GemmProblem& p);
+
+// Picks the accepted implementation with the highest score (first registered wins
+// ties), honouring the handle's forced implementation when it names a GEMM of
+// this kind. Throws NOT_SUPPORTED when nothing accepts.
+const icl_gemm_variant* gemm_select(iclblasContext* h, const GemmProblem& p);
+
+// ---- algorithms ----
+// C = alpha*op(A)*op(B) + beta*C (real). mask: ICL_MASK_* applied to the stored C.
+void sgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int k, float alpha, const float* A,
+               long lda, const float* B, long ldb, float beta, float* C, long ldc, int mask = ICL_MASK_NONE,
+               bool record = true);
+
+void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int k, const float alpha[2],
+               const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc);
+
+// Blocked STRSM, in place on B.
+void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
+               const float* A, long lda, float* B, long ldb, bool record = true);
+
+// STRSM + FP64-residual iterative refinement. ws0/ws1: m*n floats each.
+void strsm_refined_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
+                       const float* A, long lda, float* B, long ldb, int sweeps, float* ws0, float* ws1);
+
+void sgemm_backward_dev(iclblasContext* h, int transa, int transb, int m, int n, int k, float alpha, const float* A,
+                        long lda, const float* B, long ldb, const float* dC, long lddc, float* dA, long ldda,
+                        float* dB, long lddb);
+
+// G workspace: m*n floats.
+void strsm_backward_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
+                        const float* A, long lda, const float* X, long ldx, const float* dX, long lddx, float* dA,
+                        long ldda, float* dB, long lddb, float* G);
+
+} // namespace icl

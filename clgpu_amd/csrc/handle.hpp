@@ -1,0 +1,89 @@
+// Library context behind iclblasHandle_t.
+//
+// Replaces the reference's iclblasContext + iclgpu::context + OpenCL toolkit
+// (libs/blas/shlib/iclBLASImpl.hpp:25-35, libs/blas/shlib/iclBLAS.cpp:21-39,
+// core/src/ocl/ocl_toolkit.cpp:92-191): instead of an OpenCL context, a queue and
+// a JIT program cache, a handle holds one HIP device, one HIP stream, the
+// memory/math modes, an implementation override and grow-only device scratch
+// buffers used to stage host matrices and hold refinement temporaries.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "iclBLAS.h"
+#include "iclBLAS_ext.h"
+
+namespace icl {
+
+// Error carrying an iclblasStatus_t across the C++ layer; converted back to a
+// status at the C boundary (nothing is thrown across extern "C").
+struct status_error : std::runtime_error {
+    iclblasStatus_t status;
+    status_error(iclblasStatus_t s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        throw status_error(ICLBLAS_STATUS_ALLOC_FAILED, std::string(what) + ": " + hipGetErrorString(e));
+    throw status_error(ICLBLAS_STATUS_EXECUTION_FAILED, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// A grow-only device allocation owned by the handle.
+class DeviceScratch {
+public:
+    DeviceScratch() = default;
+    DeviceScratch(const DeviceScratch&) = delete;
+    DeviceScratch& operator=(const DeviceScratch&) = delete;
+    ~DeviceScratch() { release(); }
+
+    // Returns at least `bytes` of device memory. Growing waits for `stream` first
+    // because queued work may still use the old allocation.
+    void* get(size_t bytes, hipStream_t stream)
+    {
+        if (bytes <= cap_ && ptr_) return ptr_;
+        if (ptr_) {
+            hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            release();
+        }
+        const size_t want = bytes < 256 ? 256 : bytes;
+        hip_check(hipMalloc(&ptr_, want), "hipMalloc");
+        cap_ = want;
+        return ptr_;
+    }
+
+    void release()
+    {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        cap_ = 0;
+    }
+
+private:
+    void* ptr_ = nullptr;
+    size_t cap_ = 0;
+};
+
+} // namespace icl
+
+struct iclblasContext {
+    static constexpr int kTag = 0xB1A5;
+    static constexpr int kScratchSlots = 6;
+
+    int tag = kTag;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    iclblasMemoryMode_t memory_mode = ICLBLAS_MEMORY_HOST;
+    iclblasMathMode_t math_mode = ICLBLAS_MATH_FP32;
+    std::string forced_impl;
+    std::string last_impl;
+    std::mutex mu; // a handle serializes its own calls
+    icl::DeviceScratch scratch[kScratchSlots];
+
+    static bool valid(const iclblasContext* h) { return h != nullptr && h->tag == kTag; }
+};

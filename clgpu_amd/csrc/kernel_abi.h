@@ -1,0 +1,108 @@
+/*
+ * kernel_abi.h -- the thin C ABI between the C++ host layer (handle, validation,
+ * implementation registry, memory staging) and the gfx950 kernels.
+ *
+ * Everything here works on device pointers and a HIP stream and returns a
+ * hipError_t; nothing allocates. This replaces the reference's engine layer
+ * (include/iclgpu/engine.hpp:72-423, core/src/ocl/): kernels are compiled ahead
+ * of time for gfx950 and looked up in static tables instead of being JIT-built
+ * from a string database.
+ */
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct icl_gemm_args {
+    int M, N, K;
+    const float* A; /* complex kernels: interleaved {re, im} */
+    const float* B;
+    float* C;
+    long lda, ldb, ldc; /* in elements of the data type */
+    float alpha_re, alpha_im, beta_re, beta_im;
+    int mask; /* output triangle mask, ICL_MASK_* */
+} icl_gemm_args;
+
+enum {
+    ICL_MASK_NONE = 0,
+    ICL_MASK_LOWER = 1,        /* keep m >= n, zero elsewhere */
+    ICL_MASK_LOWER_STRICT = 2, /* keep m >  n */
+    ICL_MASK_UPPER = 3,        /* keep m <= n */
+    ICL_MASK_UPPER_STRICT = 4  /* keep m <  n */
+};
+
+enum { ICL_KIND_SGEMM = 0, ICL_KIND_CGEMM = 1 };
+enum { ICL_MATH_FP32 = 0, ICL_MATH_BF16X3 = 1 };
+
+/* One compiled GEMM implementation (a tile configuration with all its
+ * op/edge instantiations). opa/opb: 0 = N, 1 = T, 2 = C. edge = 0 requires
+ * M % bm == N % bn == K % bk == 0 and 16-byte aligned operand columns. */
+typedef struct icl_gemm_variant {
+    const char* name;
+    int kind;
+    int math;
+    int bm, bn, bk;
+    int threads;
+    hipError_t (*launch)(const icl_gemm_args* args, int opa, int opb, int edge, hipStream_t stream);
+} icl_gemm_variant;
+
+const icl_gemm_variant* icl_sgemm_f32_variants(int* count);
+const icl_gemm_variant* icl_sgemm_bf16x3_variants(int* count);
+const icl_gemm_variant* icl_cgemm_f32_variants(int* count);
+const icl_gemm_variant* icl_cgemm_bf16x3_variants(int* count);
+
+/* Triangular solve of one diagonal block (at most 64 x 64) in canonical
+ * forward-substitution form L * X = s_pre * X0 (L lower triangular), with
+ * L(i, k) = L[i * l_si + k * l_sk] and X(i, j) = X[i * x_si + j * x_sj];
+ * strides may be negative (backward substitution and right-side solves are
+ * expressed by reversing / transposing the index maps). Each of the nlines
+ * columns j is independent. The solution is scaled by s_post on store. */
+typedef struct icl_trsm_diag_args {
+    int nb;
+    int nlines;
+    const float* L;
+    long l_si, l_sk;
+    float* X;
+    long x_si, x_sj;
+    int unit;
+    float s_pre, s_post;
+} icl_trsm_diag_args;
+hipError_t icl_trsm_diag(const icl_trsm_diag_args* a, hipStream_t stream);
+
+/* R = fp32(alpha * B0 - P * Q) with the products and the sum in FP64.
+ * P is M x K, Q is K x N (fp32, column-major, optionally transposed); the
+ * operand selected by tri_on (1 = P, 2 = Q) is triangular: entries outside
+ * the triangle (lower if tri_lower) are taken as 0 and, if tri_unit, the
+ * diagonal as 1. */
+typedef struct icl_residual_args {
+    int M, N, K;
+    const float* P;
+    long ldp;
+    int p_trans;
+    const float* Q;
+    long ldq;
+    int q_trans;
+    const float* B0;
+    long ldb0;
+    float* R;
+    long ldr;
+    double alpha;
+    int tri_on;
+    int tri_lower;
+    int tri_unit;
+} icl_residual_args;
+hipError_t icl_residual_f64(const icl_residual_args* a, hipStream_t stream);
+
+/* Y(m x n) = alpha * X + beta * Y elementwise (beta == 0: Y not read;
+ * X may be NULL when alpha == 0). */
+hipError_t icl_axpby_2d(int m, int n, float alpha, const float* X, long ldx, float beta, float* Y, long ldy,
+                        hipStream_t stream);
+/* Complex version of the same (used for CGEMM with k == 0 or alpha == 0). */
+hipError_t icl_caxpby_2d(int m, int n, float alpha_re, float alpha_im, const float* X, long ldx, float beta_re,
+                         float beta_im, float* Y, long ldy, hipStream_t stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif

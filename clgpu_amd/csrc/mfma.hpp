@@ -1,0 +1,85 @@
+// Device-side building blocks for the gfx950 (CDNA4) level-3 kernels:
+// vector types, MFMA wrappers, the FP32 -> BF16 hi/lo split and the
+// XCD-aware workgroup -> tile mapping.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+namespace icl {
+
+// v_mfma_f32_32x32x2_f32. Lane l supplies a = A[l%32][l/32] (32x2 operand) and
+// b = B[l/32][l%32] (2x32 operand). D[i][j] for i = 8*(r/4) + 4*(l/32) + r%4,
+// j = l%32 is register r of lane l.
+__device__ __forceinline__ f32x16 mfma_32x32x2_f32(float a, float b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// v_mfma_f32_32x32x16_bf16. Lane l supplies A[l%32][8*(l/32) .. +7] and
+// B[8*(l/32) .. +7][l%32]; D has the same lane layout as the f32 32x32 form.
+__device__ __forceinline__ f32x16 mfma_32x32x16_bf16(bf16x8 a, bf16x8 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// v_mfma_f64_16x16x4_f64. Lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// D[i][j] for i = 4*(l/16) + r, j = l%16 is register r of lane l.
+__device__ __forceinline__ f64x4 mfma_16x16x4_f64(double a, double b, f64x4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Row index (within a 32x32 MFMA block) held by register r of lane `lane`.
+__device__ __forceinline__ int mfma32_row(int lane, int r)
+{
+    return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+}
+
+// x = hi + lo + e with hi = bf16(x), lo = bf16(x - hi) (both round-to-nearest-even)
+// and |e| <= 2^-17 |x| for normal x. x - hi is exact in FP32. Non-finite x
+// gives lo = 0 so that inf stays inf instead of turning into inf - inf = NaN.
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo)
+{
+    hi = (__bf16)x;
+    float r = x - (float)hi;
+    lo = (__bf16)(__builtin_isfinite(x) ? r : 0.0f);
+}
+
+__device__ __forceinline__ bf16x8 bf16x8_neg(bf16x8 v)
+{
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+// Maps blockIdx.x to a (tile_m, tile_n) pair.
+// 1) The dispatcher hands consecutive workgroup ids to the 8 XCDs round-robin,
+//    and each XCD has a private L2. Renumbering id -> (id % 8) * (n / 8) + id / 8
+//    gives every XCD a contiguous range of logical ids.
+// 2) Logical ids are rastered in groups of GROUP tile-rows so that workgroups
+//    running together on one XCD share A row-panels and B column-panels in L2.
+template <int GROUP>
+__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn)
+{
+    const int nb = gridDim.x;
+    int b = blockIdx.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    const int per_group = GROUP * tiles_n;
+    const int g = b / per_group;
+    const int first = g * GROUP;
+    const int gsz = min(tiles_m - first, GROUP);
+    const int r = b - g * per_group;
+    tm = first + r % gsz;
+    tn = r / gsz;
+}
+
+} // namespace icl

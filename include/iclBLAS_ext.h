@@ -1,0 +1,99 @@
+/*
+ * iclBLAS_ext.h -- MI355X extensions to the iclBLAS C API.
+ *
+ * None of these entry points exists in intel/clGPU; they are additions that
+ * the reference API has no room for. All are plain C: pointers, ints and enums.
+ *
+ *  - Memory mode: by default (ICLBLAS_MEMORY_HOST) matrix arguments are host
+ *    pointers and each call is synchronous, as in the reference
+ *    (libs/blas/shlib/iclBLASImpl.hpp:236-245). ICLBLAS_MEMORY_DEVICE takes
+ *    device pointers (hipMalloc / torch CUDA tensors), launches on the
+ *    handle's HIP stream and returns without waiting. Scalars (alpha, beta)
+ *    are host pointers in both modes.
+ *  - Math mode: ICLBLAS_MATH_FP32 runs SGEMM/CGEMM on FP32 MFMA.
+ *    ICLBLAS_MATH_SPLIT_BF16X3 splits every FP32 operand x into
+ *    hi = bf16(x), lo = bf16(x - hi) and forms hi*hi + hi*lo + lo*hi on BF16
+ *    MFMA with FP32 accumulation.
+ *  - Refined STRSM: FP32 blocked solve followed by sweeps of iterative
+ *    refinement whose residual R = alpha*B - op(A)*X is formed in FP64.
+ *  - Adjoints of SGEMM and STRSM.
+ *  - Implementation override: run a named kernel variant, bypassing the
+ *    accept/score selection (the reference's tests do the same through
+ *    tests/blas_funcs/test_helpers.hpp:78-102).
+ */
+#pragma once
+#include "iclBLAS.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ICLBLAS_MEMORY_HOST = 0,
+    ICLBLAS_MEMORY_DEVICE = 1
+} iclblasMemoryMode_t;
+
+typedef enum {
+    ICLBLAS_MATH_FP32 = 0,
+    ICLBLAS_MATH_SPLIT_BF16X3 = 1
+} iclblasMathMode_t;
+
+ICLBLAS_API iclblasStatus_t iclblasSetMemoryMode(iclblasHandle_t handle, iclblasMemoryMode_t mode);
+ICLBLAS_API iclblasStatus_t iclblasGetMemoryMode(iclblasHandle_t handle, iclblasMemoryMode_t* mode);
+ICLBLAS_API iclblasStatus_t iclblasSetMathMode(iclblasHandle_t handle, iclblasMathMode_t mode);
+ICLBLAS_API iclblasStatus_t iclblasGetMathMode(iclblasHandle_t handle, iclblasMathMode_t* mode);
+
+/* Stream used for every launch of this handle; `stream` is a hipStream_t
+ * (NULL = the null stream). The handle does not own a stream set this way. */
+ICLBLAS_API iclblasStatus_t iclblasSetStream(iclblasHandle_t handle, void* stream);
+ICLBLAS_API iclblasStatus_t iclblasGetStream(iclblasHandle_t handle, void** stream);
+/* Waits for all work queued on the handle's stream. */
+ICLBLAS_API iclblasStatus_t iclblasSynchronize(iclblasHandle_t handle);
+
+/* Forces the named implementation for subsequent calls of the function it
+ * belongs to (NULL or "" restores automatic selection). A call whose
+ * parameters the forced implementation does not accept returns
+ * ICLBLAS_STATUS_NOT_SUPPORTED. */
+ICLBLAS_API iclblasStatus_t iclblasSetImplementation(iclblasHandle_t handle, const char* name);
+/* Copies the name of the implementation that ran last into buf (NUL-terminated). */
+ICLBLAS_API iclblasStatus_t iclblasGetLastImplementation(iclblasHandle_t handle, char* buf, int len);
+
+/* STRSM with mixed-precision iterative refinement: X0 = FP32 solve; then for
+ * each sweep R = alpha*B - op(A)*X in FP64 on the GPU, D = FP32 solve of R,
+ * X += D. Same arguments and validation as iclblasStrsm plus
+ * 0 <= max_sweeps <= 8. max_sweeps == 0 is the plain solve. */
+ICLBLAS_API iclblasStatus_t iclblasStrsmRefined(iclblasHandle_t handle, iclblasSideMode_t side,
+                                                iclblasFillMode_t uplo, iclblasOperation_t trans,
+                                                iclblasDiagType_t diag, int m, int n, const float* alpha,
+                                                float* A, int lda, float* B, int ldb, int max_sweeps);
+
+/* Adjoint of C = alpha*op(A)*op(B) + beta*C for an upstream gradient dC (m x n):
+ *   dA (shape of A) = alpha * dC * op(B)^T      (transa == N)
+ *                   = alpha * op(B) * dC^T      (transa != N)
+ *   dB (shape of B) = alpha * op(A)^T * dC      (transb == N)
+ *                   = alpha * dC^T * op(A)      (transb != N)
+ * dA or dB may be NULL to skip it. Both are overwritten. */
+ICLBLAS_API iclblasStatus_t iclblasSgemmBackward(iclblasHandle_t handle, iclblasOperation_t transa,
+                                                 iclblasOperation_t transb, int m, int n, int k,
+                                                 const float* alpha, float* A, int lda, float* B, int ldb,
+                                                 float* dC, int lddc, float* dA, int ldda, float* dB,
+                                                 int lddb);
+
+/* Adjoint of X = alpha * op(A)^-1 * B (left) or X = alpha * B * op(A)^-1 (right),
+ * given the forward output X (m x n) and the upstream gradient dX (m x n):
+ *   G  = op(A)^-T * dX   (left)   or   dX * op(A)^-T   (right)
+ *   dB = alpha * G
+ *   dA = tri(-G * X^T) (left)  or  tri(-X^T * G) (right), transposed when
+ *        trans != N; tri() keeps the stored triangle of A (uplo) and drops
+ *        the diagonal when diag == UNIT. Entries of dA outside the triangle
+ *        are set to 0.
+ * dA or dB may be NULL to skip it. */
+ICLBLAS_API iclblasStatus_t iclblasStrsmBackward(iclblasHandle_t handle, iclblasSideMode_t side,
+                                                 iclblasFillMode_t uplo, iclblasOperation_t trans,
+                                                 iclblasDiagType_t diag, int m, int n, const float* alpha,
+                                                 float* A, int lda, float* X, int ldx, float* dX, int lddx,
+                                                 float* dA, int ldda, float* dB, int lddb);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif

@@ -1,0 +1,201 @@
+"""CPU oracle for the level-3 hot path -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, by __graft_entry__.smoke() (as the checker) and by
+bench.py's CPU-baseline leg. The library (clgpu_amd) never imports it.
+
+liboracle.so (oracle.c) restates the reference's CPU semantics:
+  sgemm/cgemm  <- cpu_gemm, reference tests/blas_funcs/test_gemm.hpp:83-201
+  strsm        <- Strsm_naive, reference libs/blas/funcs/Strsm/Strsm_naive.cl:18-126
+and provides FP64 references (dgemm, zgemm, dtrsm) for error reporting.
+_ref/libref_gemm.so, when present, is the reference's own cpu_gemm compiled from
+its headers (oracle/Makefile, target `ref`).
+
+All matrices are numpy arrays in Fortran (column-major) order whose shape[0]
+is the leading dimension.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libref_gemm.so"
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="F_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_lib = None
+_ref = None
+
+
+def build(reference: str | None = None) -> None:
+    """Compile liboracle.so (and oracle/_ref when the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+    ref = reference or "/root/reference"
+    if os.path.isdir(os.path.join(ref, "tests", "blas_funcs")):
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REFERENCE={ref}"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ct.CDLL(str(LIB))
+        I, F, D, P = ct.c_int, ct.c_float, ct.c_double, ct.c_void_p
+        L.orc_sgemm.argtypes = [I, I, I, I, I, F, P, I, P, I, F, P, I]
+        L.orc_sgemm_mt.argtypes = [I, I, I, I, I, I, F, P, I, P, I, F, P, I]
+        L.orc_sgemm_mt.restype = I
+        L.orc_cgemm.argtypes = [I, I, I, I, I, P, P, I, P, I, P, P, I]
+        L.orc_strsm.argtypes = [I, I, I, I, I, I, F, P, I, P, I]
+        L.orc_dtrsm.argtypes = [I, I, I, I, I, I, D, P, I, P, I]
+        L.orc_dgemm.argtypes = [I, I, I, I, I, D, P, I, P, I, D, P, I, P, P]
+        L.orc_dgemm_sample.argtypes = [I, I, I, D, P, I, P, I, D, P, I, I, P, P, P, P]
+        L.orc_zgemm.argtypes = [I, I, I, I, I, P, P, I, P, I, P, P, I, P, P]
+        L.orc_zgemm_sample.argtypes = [I, I, I, P, P, I, P, I, P, P, I, I, P, P, P, P]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return REF_LIB.exists()
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        R = ct.CDLL(str(REF_LIB))
+        I, F, P = ct.c_int, ct.c_float, ct.c_void_p
+        R.ref_cpu_sgemm.argtypes = [I, I, I, I, I, F, P, I, P, I, F, P, I]
+        R.ref_cpu_cgemm.argtypes = [I, I, I, I, I, P, P, I, P, I, P, P, I]
+        _ref = R
+    return _ref
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def _chk(a: np.ndarray, dtype) -> np.ndarray:
+    assert a.dtype == dtype and a.flags.f_contiguous, (a.dtype, a.flags)
+    return a
+
+
+def _ld(a: np.ndarray) -> int:
+    return max(1, a.shape[0])
+
+
+def op_shape(trans: int, rows: int, cols: int):
+    return (rows, cols) if trans == 0 else (cols, rows)
+
+
+# ---------------------------------------------------------------- layer 1
+def sgemm(transa, transb, m, n, k, alpha, A, B, beta, C, reference=False):
+    """C <- cpu_gemm(...) in place (C float32 Fortran array)."""
+    _chk(A, np.float32), _chk(B, np.float32), _chk(C, np.float32)
+    fn = ref_lib().ref_cpu_sgemm if reference else lib().orc_sgemm
+    fn(transa, transb, m, n, k, alpha, _p(A), _ld(A), _p(B), _ld(B), beta, _p(C), _ld(C))
+    return C
+
+
+def sgemm_mt(threads, transa, transb, m, n, k, alpha, A, B, beta, C):
+    _chk(A, np.float32), _chk(B, np.float32), _chk(C, np.float32)
+    return lib().orc_sgemm_mt(threads, transa, transb, m, n, k, alpha, _p(A), _ld(A), _p(B), _ld(B), beta, _p(C),
+                              _ld(C))
+
+
+def cgemm(transa, transb, m, n, k, alpha, A, B, beta, C, reference=False):
+    _chk(A, np.complex64), _chk(B, np.complex64), _chk(C, np.complex64)
+    al = np.array([alpha.real, alpha.imag], np.float32)
+    be = np.array([beta.real, beta.imag], np.float32)
+    fn = ref_lib().ref_cpu_cgemm if reference else lib().orc_cgemm
+    fn(transa, transb, m, n, k, _p(al), _p(A), _ld(A), _p(B), _ld(B), _p(be), _p(C), _ld(C))
+    return C
+
+
+def strsm(side, uplo, trans, diag, m, n, alpha, A, B):
+    _chk(A, np.float32), _chk(B, np.float32)
+    lib().orc_strsm(side, uplo, trans, diag, m, n, alpha, _p(A), _ld(A), _p(B), _ld(B))
+    return B
+
+
+# ---------------------------------------------------------------- layer 2
+def dgemm(transa, transb, m, n, k, alpha, A, B, beta=0.0, C=None):
+    """Returns (C64, absAB) as dense m x n float64 (C order irrelevant: Fortran view)."""
+    _chk(A, np.float32), _chk(B, np.float32)
+    C64 = np.zeros(m * n, np.float64)
+    ab = np.zeros(m * n, np.float64)
+    Cp, ldc = (_p(C), _ld(C)) if C is not None else (None, 1)
+    lib().orc_dgemm(transa, transb, m, n, k, float(alpha), _p(A), _ld(A), _p(B), _ld(B), float(beta), Cp, ldc,
+                    _p(C64), _p(ab))
+    return C64.reshape(n, m).T, ab.reshape(n, m).T
+
+
+def dgemm_sample(transa, transb, k, alpha, A, B, rows, cols, beta=0.0, C=None):
+    _chk(A, np.float32), _chk(B, np.float32)
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    out = np.zeros(len(rows), np.float64)
+    ab = np.zeros(len(rows), np.float64)
+    Cp, ldc = (_p(C), _ld(C)) if C is not None else (None, 1)
+    lib().orc_dgemm_sample(transa, transb, k, float(alpha), _p(A), _ld(A), _p(B), _ld(B), float(beta), Cp, ldc,
+                           len(rows), _p(rows), _p(cols), _p(out), _p(ab))
+    return out, ab
+
+
+def zgemm(transa, transb, m, n, k, alpha, A, B, beta=0.0, C=None):
+    _chk(A, np.complex64), _chk(B, np.complex64)
+    al = np.array([complex(alpha).real, complex(alpha).imag], np.float64)
+    be = np.array([complex(beta).real, complex(beta).imag], np.float64)
+    out = np.zeros(2 * m * n, np.float64)
+    ab = np.zeros(m * n, np.float64)
+    Cp, ldc = (_p(C), _ld(C)) if C is not None else (None, 1)
+    lib().orc_zgemm(transa, transb, m, n, k, _p(al), _p(A), _ld(A), _p(B), _ld(B), _p(be), Cp, ldc, _p(out), _p(ab))
+    z = out[0::2] + 1j * out[1::2]
+    return z.reshape(n, m).T, ab.reshape(n, m).T
+
+
+def zgemm_sample(transa, transb, k, alpha, A, B, rows, cols, beta=0.0, C=None):
+    _chk(A, np.complex64), _chk(B, np.complex64)
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    al = np.array([complex(alpha).real, complex(alpha).imag], np.float64)
+    be = np.array([complex(beta).real, complex(beta).imag], np.float64)
+    out = np.zeros(2 * len(rows), np.float64)
+    ab = np.zeros(len(rows), np.float64)
+    Cp, ldc = (_p(C), _ld(C)) if C is not None else (None, 1)
+    lib().orc_zgemm_sample(transa, transb, k, _p(al), _p(A), _ld(A), _p(B), _ld(B), _p(be), Cp, ldc, len(rows),
+                           _p(rows), _p(cols), _p(out), _p(ab))
+    return out[0::2] + 1j * out[1::2], ab
+
+
+def dtrsm(side, uplo, trans, diag, m, n, alpha, A, B):
+    """FP64 substitution; B (float32) is read, a float64 m x n result returned."""
+    _chk(A, np.float32)
+    X = np.asfortranarray(np.array(B[:m, :n], dtype=np.float64))
+    lib().orc_dtrsm(side, uplo, trans, diag, m, n, float(alpha), _p(A), _ld(A), _p(X), _ld(X))
+    return X
+
+
+# ---------------------------------------------------------------- error metrics
+U32 = 2.0 ** -24
+
+
+def gemm_errors(Cg, C64, absAB, k, beta_absC=None):
+    """Max |err|, max relative error and the max ratio to the elementwise bound
+    4*k*2^-24*(|A||B|)_ij (reported as err / (k*2^-24*(|A||B|)_ij), must be <= 4).
+    beta_absC adds |beta||C|*2^-24 to the bound denominator for beta != 0."""
+    err = np.abs(np.asarray(Cg, np.complex128 if np.iscomplexobj(Cg) else np.float64) - C64)
+    den = k * U32 * np.asarray(absAB, np.float64)
+    if beta_absC is not None:
+        den = den + U32 * beta_absC
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = np.where(den > 0, err / den, np.where(err == 0, 0.0, np.inf))
+        rel = np.where(np.abs(C64) > 0, err / np.abs(C64), np.where(err == 0, 0.0, np.inf))
+    return {"max_abs_err": float(err.max(initial=0.0)), "max_rel_err": float(rel.max(initial=0.0)),
+            "max_bound_ratio": float(ratio.max(initial=0.0))}

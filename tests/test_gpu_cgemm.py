@@ -1,0 +1,131 @@
+"""CGEMM parity on the GPU through the C ABI: the reference's known answers
+(tests/iclBLAS/test_Cgemm.cpp, 4 ULP per component), the implementation-level
+grid the reference left disabled (tests/blas_funcs/test_gemm.cpp:102-122:
+N/T/C x N/T/C, padded leading dimensions, beta zero / non-zero) on
+quarter-integer data with exact equality against cpu_gemm<complex<float>>,
+and the FP64 bound on random data, for the FP32 and split-BF16 paths."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from blas_helpers import U32, bound_ratio, cmat, complex_eq, flat
+from clgpu_amd import datagen
+from clgpu_amd import iclblas as ib
+
+pytestmark = pytest.mark.gpu
+KATS = json.loads((Path(__file__).parent / "golden" / "kats.json").read_text())
+MODES = [ib.MATH_FP32, ib.MATH_SPLIT_BF16X3]
+IMPLS = ["cgemm_f32_mfma_64x64", "cgemm_f32_mfma_128x128", "cgemm_bf16x3_mfma_64x64", "cgemm_bf16x3_mfma_128x128"]
+
+
+@pytest.fixture
+def h(handle):
+    handle.set_math_mode(ib.MATH_FP32)
+    handle.set_implementation(None)
+    yield handle
+    handle.set_math_mode(ib.MATH_FP32)
+    handle.set_implementation(None)
+
+
+def run(h, ta, tb, m, n, k, alpha, A, B, beta, C):
+    ib.check(ib.iclblasCgemm(h.value, ta, tb, m, n, k, alpha, A, A.shape[0], B, B.shape[0], beta, C, C.shape[0]),
+             "iclblasCgemm")
+    return C
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("case", KATS["cgemm"], ids=lambda c: c["name"])
+def test_cgemm_kat(h, case, mode):
+    c = case
+    h.set_math_mode(mode)
+    A = cmat(c["A"], c["lda"], c["k"] if c["transa"] == 0 else c["m"])
+    B = cmat(c["B"], c["ldb"], c["n"] if c["transb"] == 0 else c["k"])
+    C = cmat(c["C"], c["ldc"], c["n"])
+    run(h, c["transa"], c["transb"], c["m"], c["n"], c["k"], complex(*c["alpha"]), A, B, complex(*c["beta"]), C)
+    exp = np.asarray(c["expected"], np.float64)
+    assert complex_eq(flat(C), exp[:, 0] + 1j * exp[:, 1])
+
+
+def _qcmat(rows, cols, ld, stream):
+    v = datagen.quarter_integers(2 * ld * cols, seed=13, stream=stream)
+    z = (v[0::2] + 1j * v[1::2]).astype(np.complex64)
+    return np.asfortranarray(z.reshape(cols, ld).T)
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+@pytest.mark.parametrize("ta", [0, 1, 2])
+@pytest.mark.parametrize("tb", [0, 1, 2])
+@pytest.mark.parametrize("beta_zero", [False, True])
+def test_cgemm_quarter_integer_grid_exact(h, impl, ta, tb, beta_zero):
+    m, n, k = 96, 80, 112
+    pad = 29 if (ta + tb) % 2 else 0
+    ra, ca = (m, k) if ta == 0 else (k, m)
+    rb, cb = (k, n) if tb == 0 else (n, k)
+    A = _qcmat(ra, ca, ra + pad, 0)
+    B = _qcmat(rb, cb, rb + pad, 1)
+    C = _qcmat(m, n, m + pad, 2)
+    alpha = 0.75 - 0.5j
+    beta = 0.0 if beta_zero else -0.25 + 1.0j
+    if beta_zero:
+        C[:m] = np.nan
+    ref = C.copy(order="F")
+    oracle.cgemm(ta, tb, m, n, k, alpha, A, B, beta, ref)
+    h.set_implementation(impl)
+    run(h, ta, tb, m, n, k, alpha, A, B, beta, C)
+    assert h.last_implementation() == impl
+    assert np.array_equal(C, ref, equal_nan=True)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("ta,tb", [(0, 0), (2, 0), (1, 2), (2, 2), (0, 1)])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (37, 21, 53), (130, 70, 200)], ids=lambda s: "x".join(map(str, s)))
+def test_cgemm_error_bound(h, mode, ta, tb, shape):
+    m, n, k = shape
+    h.set_math_mode(mode)
+    ra, ca = (m, k) if ta == 0 else (k, m)
+    rb, cb = (k, n) if tb == 0 else (n, k)
+    A = datagen.complex_matrix(ra, ca, ra + 1, "A", seed=31)
+    B = datagen.complex_matrix(rb, cb, rb + 2, "B", seed=31)
+    C = datagen.complex_matrix(m, n, m + 1, "C", seed=31)
+    alpha, beta = 1 + 0.5j, 0.25
+    C64, ab = oracle.zgemm(ta, tb, m, n, k, alpha, A, B, beta, C)
+    C0 = C.copy(order="F")
+    run(h, ta, tb, m, n, k, alpha, A, B, beta, C)
+    err = np.abs(C[:m].astype(np.complex128) - C64)
+    extra = 4 * U32 * (abs(beta) * np.abs(C0[:m]).astype(np.float64) + abs(alpha) * ab)
+    assert bound_ratio(err, k, abs(alpha) * ab, extra) <= 4.0
+    assert np.array_equal(C[m:], C0[m:])
+
+
+def _to_dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("ta", [0, 2])
+def test_cgemm_config3_4096(dev_handle, mode, ta):
+    """CGEMM 4096^3, NN and CN, alpha = (1, 0.5), beta = (0.25, 0)."""
+    import torch
+    n = 4096
+    A = datagen.complex_matrix(n, n, name="A", seed=0)
+    B = datagen.complex_matrix(n, n, name="B", seed=0)
+    C = datagen.complex_matrix(n, n, name="C", seed=0)
+    h = dev_handle
+    h.set_math_mode(mode)
+    dA, dB, dC = _to_dev(A), _to_dev(B), _to_dev(C)
+    alpha, beta = 1 + 0.5j, 0.25
+    ib.check(ib.iclblasCgemm(h.value, ta, 0, n, n, n, alpha, dA.data_ptr(), n, dB.data_ptr(), n, beta, dC.data_ptr(),
+                             n))
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(ta)
+    rows = rng.integers(0, n, 256)
+    cols = rng.integers(0, n, 256)
+    got = dC.t()[torch.from_numpy(rows).cuda(), torch.from_numpy(cols).cuda()].cpu().numpy().astype(np.complex128)
+    c64, ab = oracle.zgemm_sample(ta, 0, n, alpha, A, B, rows, cols, beta, C)
+    extra = 4 * U32 * (abs(beta) * np.abs(C[rows, cols]).astype(np.float64) + abs(alpha) * ab)
+    assert bound_ratio(np.abs(got - c64), n, abs(alpha) * ab, extra) <= 4.0
+    h.set_math_mode(ib.MATH_FP32)

@@ -1,0 +1,223 @@
+"""SGEMM parity on the GPU through the C ABI (libiclBLAS.so), against the
+oracle: the reference's known answers, its implementation-level grid
+(tests/blas_funcs/test_gemm.cpp:56-72, quarter-integer data, exact equality),
+and the FP64 elementwise bound |C - C64|_ij <= 4*k*2^-24*(|A||B|)_ij for
+random data, on the plain FP32-MFMA and the split-BF16 paths."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from blas_helpers import U32, bound_ratio, flat, float_eq, fmat
+from clgpu_amd import datagen
+from clgpu_amd import iclblas as ib
+
+pytestmark = pytest.mark.gpu
+KATS = json.loads((Path(__file__).parent / "golden" / "kats.json").read_text())
+MODES = [ib.MATH_FP32, ib.MATH_SPLIT_BF16X3]
+F32_IMPLS = ["sgemm_f32_mfma_128x128", "sgemm_f32_mfma_256x256", "sgemm_f32_mfma_256x256w8"]
+BF16_IMPLS = ["sgemm_bf16x3_mfma_128x128", "sgemm_bf16x3_mfma_256x128"]
+
+
+@pytest.fixture
+def h(handle):
+    handle.set_math_mode(ib.MATH_FP32)
+    handle.set_implementation(None)
+    yield handle
+    handle.set_math_mode(ib.MATH_FP32)
+    handle.set_implementation(None)
+
+
+def run(h, ta, tb, m, n, k, alpha, A, B, beta, C):
+    ib.check(ib.iclblasSgemm(h.value, ta, tb, m, n, k, alpha, A, A.shape[0], B, B.shape[0], beta, C, C.shape[0]),
+             "iclblasSgemm")
+    return C
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("case", KATS["sgemm"], ids=lambda c: c["name"])
+def test_sgemm_kat(h, case, mode):
+    c = case
+    h.set_math_mode(mode)
+    A = fmat(c["A"], c["lda"], c["k"] if c["transa"] == 0 else c["m"])
+    B = fmat(c["B"], c["ldb"], c["n"] if c["transb"] == 0 else c["k"])
+    C = fmat(c["C"], c["ldc"], c["n"])
+    C0 = C.copy(order="F")
+    run(h, c["transa"], c["transb"], c["m"], c["n"], c["k"], c["alpha"], A, B, c["beta"], C)
+    exp = np.asarray(c["expected"], np.float64)
+    if c["compare"] == "ulp4":
+        assert float_eq(flat(C), exp)
+    else:
+        _, ab = oracle.dgemm(c["transa"], c["transb"], c["m"], c["n"], c["k"], 1.0, A, B)
+        err = np.abs(C[: c["m"]].astype(np.float64) - exp.reshape(c["n"], c["ldc"]).T[: c["m"]])
+        assert bound_ratio(err, c["k"], ab, extra=U32 * np.abs(C0[: c["m"]]).astype(np.float64) * 2) <= 4.0
+
+
+# ---- the reference's implementation-level grid, every implementation forced by name
+_grid_cache = {}
+
+
+def _grid_inputs(lda_add, ldb_add, ldc_add, beta_zero):
+    key = (lda_add, ldb_add, ldc_add, beta_zero)
+    if key not in _grid_cache:
+        m = n = k = 256
+        q = lambda cnt, s: datagen.quarter_integers(cnt, seed=11, stream=s).astype(np.float32)  # noqa: E731
+        A = np.asfortranarray(q((m + lda_add) * k, 0).reshape(k, m + lda_add).T)
+        B = np.asfortranarray(q((k + ldb_add) * n, 1).reshape(n, k + ldb_add).T)
+        C = np.asfortranarray(q((m + ldc_add) * n, 2).reshape(n, m + ldc_add).T)
+        alpha = float(q(1, 3)[0]) or 0.5
+        beta = 0.0 if beta_zero else (float(q(1, 4)[0]) or -0.25)
+        if beta_zero:
+            C[:m] = np.nan  # beta == 0: C need not be initialised
+        ref = C.copy(order="F")
+        oracle.sgemm(0, 0, m, n, k, alpha, A, B, beta, ref)
+        _grid_cache[key] = (A, B, C, alpha, beta, ref)
+    return _grid_cache[key]
+
+
+@pytest.mark.parametrize("impl", F32_IMPLS + BF16_IMPLS)
+@pytest.mark.parametrize("lda_add", [0, 29])
+@pytest.mark.parametrize("ldb_add", [0, 29])
+@pytest.mark.parametrize("beta_zero", [False, True])
+@pytest.mark.parametrize("ldc_add", [0, 29])
+def test_sgemm_quarter_integer_grid_exact(h, impl, lda_add, ldb_add, beta_zero, ldc_add):
+    A, B, C, alpha, beta, ref = _grid_inputs(lda_add, ldb_add, ldc_add, beta_zero)
+    h.set_implementation(impl)
+    C = C.copy(order="F")
+    run(h, 0, 0, 256, 256, 256, alpha, A, B, beta, C)
+    assert h.last_implementation() == impl
+    assert np.array_equal(C, ref, equal_nan=True)  # padding rows untouched, values exact
+
+
+SHAPES = [(1, 1, 1), (3, 5, 7), (77, 45, 131), (128, 128, 16), (300, 257, 65), (513, 130, 1000)]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1), (2, 0), (0, 2)])
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_sgemm_error_bound(h, mode, ta, tb, shape):
+    m, n, k = shape
+    h.set_math_mode(mode)
+    ra, ca = (m, k) if ta == 0 else (k, m)
+    rb, cb = (k, n) if tb == 0 else (n, k)
+    A = datagen.matrix(ra, ca, ra + 3, "A", seed=21)
+    B = datagen.matrix(rb, cb, rb + 1, "B", seed=21)
+    C = datagen.matrix(m, n, m + 2, "C", seed=21)
+    C[m:] = -7.0  # sentinel padding
+    C0 = C.copy(order="F")
+    alpha, beta = -1.5, 0.5
+    C64, ab = oracle.dgemm(ta, tb, m, n, k, alpha, A, B, beta, C)
+    run(h, ta, tb, m, n, k, alpha, A, B, beta, C)
+    err = np.abs(C[:m].astype(np.float64) - C64)
+    extra = 2 * U32 * (abs(beta) * np.abs(C0[:m].astype(np.float64)) + abs(alpha) * ab)
+    assert bound_ratio(err, k, abs(alpha) * ab, extra) <= 4.0
+    assert np.all(C[m:] == -7.0)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_sgemm_beta_zero_does_not_read_c_and_alpha_zero_does_not_read_ab(h, mode):
+    h.set_math_mode(mode)
+    m, n, k = 70, 33, 40
+    A = datagen.matrix(m, k, name="A")
+    B = datagen.matrix(k, n, name="B")
+    C = np.full((m, n), np.nan, np.float32, order="F")
+    run(h, 0, 0, m, n, k, 1.0, A, B, 0.0, C)
+    assert np.all(np.isfinite(C))
+    An = np.full_like(A, np.nan)
+    C2 = datagen.matrix(m, n, name="C")
+    exp = (np.float32(0.25) * C2).astype(np.float32)
+    run(h, 0, 0, m, n, k, 0.0, An, An, 0.25, C2)
+    assert np.array_equal(C2, exp)
+    C3 = datagen.matrix(m, n, name="C")
+    exp3 = (np.float32(-2.0) * C3).astype(np.float32)
+    run(h, 0, 0, m, n, 0, 1.0, A, B, -2.0, C3)  # k == 0: C = beta*C
+    assert np.array_equal(C3, exp3)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_sgemm_config1_256_beta_half(h, mode):
+    """SGEMM NN 256^3, ld 256, alpha 1, beta 0.5, uniform[-1,1] from seed 0."""
+    h.set_math_mode(mode)
+    m = n = k = 256
+    A = datagen.matrix(m, k, name="A", seed=0)
+    B = datagen.matrix(k, n, name="B", seed=0)
+    C = datagen.matrix(m, n, name="C", seed=0)
+    C64, ab = oracle.dgemm(0, 0, m, n, k, 1.0, A, B, 0.5, C)
+    Cc = C.copy(order="F")
+    oracle.sgemm(0, 0, m, n, k, 1.0, A, B, 0.5, Cc)
+    run(h, 0, 0, m, n, k, 1.0, A, B, 0.5, C)
+    ratio = bound_ratio(np.abs(C - C64), k, ab, U32 * 0.5 * np.abs(Cc) * 2)
+    assert ratio <= 4.0
+    # the reference CPU path (double accumulation) is within the same bound
+    assert bound_ratio(np.abs(Cc - C64), k, ab, U32 * np.abs(Cc)) <= 4.0
+
+
+# ---- full-size shapes in device memory (sampled FP64 check + size-independent identities)
+def _to_dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+
+
+def _from_dev(t):
+    return np.asfortranarray(t.cpu().numpy().T)
+
+
+@pytest.fixture(scope="module")
+def big():
+    n = 8192
+    A = datagen.matrix(n, n, name="A", seed=0)
+    B = datagen.matrix(n, n, name="B", seed=0)
+    return A, B
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_sgemm_config2_8192(dev_handle, big, mode, ta, tb):
+    import torch
+    A, B = big
+    n = 8192
+    h = dev_handle
+    h.set_math_mode(mode)
+    dA, dB = _to_dev(A), _to_dev(B)
+    dC = torch.empty((n, n), device="cuda", dtype=torch.float32)
+    ib.check(ib.iclblasSgemm(h.value, ta, tb, n, n, n, 1.0, dA.data_ptr(), n, dB.data_ptr(), n, 0.0, dC.data_ptr(), n))
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(ta * 2 + tb)
+    rows = rng.integers(0, n, 384)
+    cols = rng.integers(0, n, 384)
+    got = dC.t()[torch.from_numpy(rows).cuda(), torch.from_numpy(cols).cuda()].cpu().numpy().astype(np.float64)
+    c64, ab = oracle.dgemm_sample(ta, tb, n, 1.0, A, B, rows, cols)
+    assert bound_ratio(np.abs(got - c64), n, ab) <= 4.0
+    h.set_math_mode(ib.MATH_FP32)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_sgemm_config2_nonsquare_padded(dev_handle, mode):
+    import torch
+    m, n, k, ld = 4096, 8192, 2048, 4160
+    A = datagen.matrix(m, k, ld, "A", seed=0)
+    B = datagen.matrix(k, n, ld, "B", seed=0)
+    h = dev_handle
+    h.set_math_mode(mode)
+    dA, dB = _to_dev(A), _to_dev(B)
+    dC = torch.full((n, ld), -3.0, device="cuda", dtype=torch.float32)
+    ib.check(ib.iclblasSgemm(h.value, 0, 0, m, n, k, 1.0, dA.data_ptr(), ld, dB.data_ptr(), ld, 0.0, dC.data_ptr(), ld))
+    torch.cuda.synchronize()
+    Ct = dC.t()
+    assert bool((Ct[m:] == -3.0).all())  # padding rows of C untouched
+    rng = np.random.default_rng(7)
+    rows = rng.integers(0, m, 384)
+    cols = rng.integers(0, n, 384)
+    got = Ct[torch.from_numpy(rows).cuda(), torch.from_numpy(cols).cuda()].cpu().numpy().astype(np.float64)
+    c64, ab = oracle.dgemm_sample(0, 0, k, 1.0, A, B, rows, cols)
+    assert bound_ratio(np.abs(got - c64), k, ab) <= 4.0
+    # linearity in B at full size: op(A)(B1 + B2) == op(A)B1 + op(A)B2 to rounding
+    dC2 = torch.empty((n, ld), device="cuda", dtype=torch.float32)
+    dB2 = dB * 0.5
+    ib.check(ib.iclblasSgemm(h.value, 0, 0, m, n, k, 2.0, dA.data_ptr(), ld, dB2.data_ptr(), ld, 0.0, dC2.data_ptr(),
+                             ld))
+    torch.cuda.synchronize()
+    assert bool((dC2.t()[:m] == Ct[:m]).all())  # 2 * (A (B/2)) is bitwise A B (power-of-two scaling)
+    h.set_math_mode(ib.MATH_FP32)
