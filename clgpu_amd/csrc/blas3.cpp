@@ -93,8 +93,15 @@ const icl_gemm_variant* gemm_select(iclblasContext* h, const GemmProblem& p)
 // ---------------------------------------------------------------------------
 // GEMM
 // ---------------------------------------------------------------------------
+// Smallest K for which the split-BF16 product meets the FP32 elementwise bound
+// |C - C64| <= 4 k 2^-24 (|A||B|): the split leaves at most 3 * 2^-16 |a||b| per
+// product (|x - hi - lo| <= 2^-16 |x| and the dropped lo*lo term), which is below
+// 3 k 2^-24 |a||b| once k >= 256. Smaller K runs on FP32 MFMA instead.
+constexpr int kSplitMinK = 256;
+
 static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, const icl_gemm_args& g, bool record)
 {
+    if (math == ICL_MATH_BF16X3 && g.K < kSplitMinK) math = ICL_MATH_FP32;
     GemmProblem p{kind, math, opa, opb, &g};
     const icl_gemm_variant* v = gemm_select(h, p);
     const bool edge = gemm_needs_edge(*v, p);

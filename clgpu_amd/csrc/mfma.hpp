@@ -32,7 +32,8 @@ __device__ __forceinline__ f32x16 mfma_32x32x16_bf16(bf16x8 a, bf16x8 b, f32x16 
 }
 
 // v_mfma_f64_16x16x4_f64. Lane l supplies A[l%16][l/16] and B[l/16][l%16];
-// D[i][j] for i = 4*(l/16) + r, j = l%16 is register r of lane l.
+// D[i][j] for i = 4*r + l/16, j = l%16 is register r of lane l (one row per
+// register, unlike the FP32 16x16 form whose registers hold 4 consecutive rows).
 __device__ __forceinline__ f64x4 mfma_16x16x4_f64(double a, double b, f64x4 c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);

@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int m = m0 + wm + 16 * i + (lane & 15);
-                const int n = n0 + wn + 16 * j + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + 16 * j + 4 * r + (lane >> 4);
                 if (m < a.M && n < a.N) {
                     const double b = (double)a.B0[(long)n * a.ldb0 + m];
                     a.R[(long)n * a.ldr + m] = (float)(a.alpha * b - acc[i][j][r]);

@@ -98,6 +98,22 @@ def test_cgemm_error_bound(h, mode, ta, tb, shape):
     extra = 4 * U32 * (abs(beta) * np.abs(C0[:m]).astype(np.float64) + abs(alpha) * ab)
     assert bound_ratio(err, k, abs(alpha) * ab, extra) <= 4.0
     assert np.array_equal(C[m:], C0[m:])
+    split = mode == ib.MATH_SPLIT_BF16X3 and k >= 256
+    assert h.last_implementation().startswith("cgemm_bf16x3" if split else "cgemm_f32")
+
+
+@pytest.mark.parametrize("impl", IMPLS[2:])
+def test_cgemm_split_forced_small_k_bound(h, impl):
+    m, n, k = 40, 24, 20
+    A = datagen.complex_matrix(m, k, name="A", seed=33)
+    B = datagen.complex_matrix(k, n, name="B", seed=33)
+    C = np.zeros((m, n), np.complex64, order="F")
+    h.set_implementation(impl)
+    run(h, 0, 0, m, n, k, 1.0, A, B, 0.0, C)
+    assert h.last_implementation() == impl
+    C64, ab = oracle.zgemm(0, 0, m, n, k, 1.0, A, B)
+    # each real product of the complex MAC carries at most 3*2^-16 |x||y|
+    assert np.all(np.abs(C - C64) <= (3 * 2.0 ** -16 + 4 * k * 2.0 ** -24) * 2 * ab)
 
 
 def _to_dev(a):

@@ -114,6 +114,25 @@ def test_sgemm_error_bound(h, mode, ta, tb, shape):
     extra = 2 * U32 * (abs(beta) * np.abs(C0[:m].astype(np.float64)) + abs(alpha) * ab)
     assert bound_ratio(err, k, abs(alpha) * ab, extra) <= 4.0
     assert np.all(C[m:] == -7.0)
+    # split-BF16 requests with k < 256 run on FP32 MFMA (the split cannot meet the bound there)
+    split = mode == ib.MATH_SPLIT_BF16X3 and k >= 256
+    assert h.last_implementation().startswith("sgemm_bf16x3" if split else "sgemm_f32")
+
+
+@pytest.mark.parametrize("impl", BF16_IMPLS)
+@pytest.mark.parametrize("shape", [(1, 1, 1), (33, 17, 16), (130, 70, 100)], ids=lambda s: "x".join(map(str, s)))
+def test_sgemm_split_forced_small_k_bound(h, impl, shape):
+    """The split kernel itself at small k: per product it leaves at most
+    3*2^-16 |a||b| (residual of the two-term split plus the dropped lo*lo term)."""
+    m, n, k = shape
+    A = datagen.matrix(m, k, name="A", seed=23)
+    B = datagen.matrix(k, n, name="B", seed=23)
+    C = np.zeros((m, n), np.float32, order="F")
+    h.set_implementation(impl)
+    run(h, 0, 0, m, n, k, 1.0, A, B, 0.0, C)
+    assert h.last_implementation() == impl
+    C64, ab = oracle.dgemm(0, 0, m, n, k, 1.0, A, B)
+    assert np.all(np.abs(C - C64) <= (3 * 2.0 ** -16 + 4 * k * U32) * ab)
 
 
 @pytest.mark.parametrize("mode", MODES)
