@@ -16,15 +16,8 @@ const std::vector<const icl_gemm_variant*>& gemm_registry()
 {
     static const std::vector<const icl_gemm_variant*> all = [] {
         std::vector<const icl_gemm_variant*> v;
-        int n = 0;
-        const icl_gemm_variant* t = icl_sgemm_f32_variants(&n);
-        for (int i = 0; i < n; ++i) v.push_back(t + i);
-        t = icl_sgemm_bf16x3_variants(&n);
-        for (int i = 0; i < n; ++i) v.push_back(t + i);
-        t = icl_cgemm_f32_variants(&n);
-        for (int i = 0; i < n; ++i) v.push_back(t + i);
-        t = icl_cgemm_bf16x3_variants(&n);
-        for (int i = 0; i < n; ++i) v.push_back(t + i);
+        const int n = icl_gemm_variant_count();
+        for (int i = 0; i < n; ++i) v.push_back(icl_gemm_variant_at(i));
         return v;
     }();
     return all;
@@ -105,7 +98,12 @@ static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, co
     GemmProblem p{kind, math, opa, opb, &g};
     const icl_gemm_variant* v = gemm_select(h, p);
     const bool edge = gemm_needs_edge(*v, p);
-    hip_check(v->launch(&g, opa, opb, edge ? 1 : 0, h->stream), v->name);
+    void* ws = nullptr;
+    if (v->workspace) {
+        const size_t bytes = v->workspace(&g);
+        if (bytes) ws = h->scratch[iclblasContext::kSlotGemmWorkspace].get(bytes, h->stream);
+    }
+    hip_check(v->launch(&g, opa, opb, edge ? 1 : 0, ws, h->stream), v->name);
     if (record) h->last_impl = v->name;
 }
 

@@ -357,6 +357,19 @@ ICLBLAS_API iclblasStatus_t iclblasSetImplementation(iclblasHandle_t handle, con
     return ICLBLAS_STATUS_INVALID_VALUE;
 }
 
+ICLBLAS_API iclblasStatus_t iclblasGetImplementations(char* buf, int len)
+{
+    if (!buf || len <= 0) return ICLBLAS_STATUS_INVALID_VALUE;
+    std::string all;
+    for (const icl_gemm_variant* v : icl::gemm_registry()) {
+        if (!all.empty()) all += '\n';
+        all += v->name;
+    }
+    if ((int)all.size() + 1 > len) return ICLBLAS_STATUS_INVALID_VALUE;
+    std::memcpy(buf, all.c_str(), all.size() + 1);
+    return ICLBLAS_STATUS_SUCCESS;
+}
+
 ICLBLAS_API iclblasStatus_t iclblasGetLastImplementation(iclblasHandle_t handle, char* buf, int len)
 {
     if (!iclblasContext::valid(handle) || !buf || len <= 0) return ICLBLAS_STATUS_INVALID_VALUE;

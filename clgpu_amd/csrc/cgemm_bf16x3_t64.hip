@@ -1,0 +1,7 @@
+// GEMM variant table (see gemm_variants.hpp).
+#include "gemm_variants.hpp"
+
+static const icl_gemm_variant kTable[] = {
+    ICL_CGEMM_BF16X3("cgemm_bf16x3_mfma_64x64", 64, 64, 16, 2, 2),
+};
+ICL_REGISTER_VARIANTS(kTable)

@@ -4,10 +4,11 @@
 
 namespace icl {
 
-// Launches `kern` over ceil(M/BM) * ceil(N/BN) workgroups. Kernels that need more
-// than 64 KiB of LDS get the dynamic-LDS limit raised once per instantiation.
-template <int BM, int BN, int NT, size_t LDS, typename Kern>
-inline hipError_t launch_gemm(Kern kern, const GemmArgs* g, hipStream_t s)
+// Launches `kern(arg)` over ceil(M/BM) * ceil(N/BN) workgroups (M, N from g).
+// Kernels that need more than 64 KiB of LDS get the dynamic-LDS limit raised
+// once per instantiation.
+template <int BM, int BN, int NT, size_t LDS, typename Kern, typename Arg>
+inline hipError_t launch_gemm(Kern kern, const GemmArgs* g, hipStream_t s, const Arg& arg)
 {
     if constexpr (LDS > 65536) {
         static const hipError_t attr =
@@ -18,8 +19,14 @@ inline hipError_t launch_gemm(Kern kern, const GemmArgs* g, hipStream_t s)
     const long tiles = (long)((g->M + BM - 1) / BM) * ((g->N + BN - 1) / BN);
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), LDS, s, *g);
+    hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), LDS, s, arg);
     return hipGetLastError();
+}
+
+template <int BM, int BN, int NT, size_t LDS, typename Kern>
+inline hipError_t launch_gemm(Kern kern, const GemmArgs* g, hipStream_t s)
+{
+    return launch_gemm<BM, BN, NT, LDS>(kern, g, s, *g);
 }
 
 } // namespace icl

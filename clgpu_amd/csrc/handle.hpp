@@ -72,7 +72,8 @@ private:
 
 struct iclblasContext {
     static constexpr int kTag = 0xB1A5;
-    static constexpr int kScratchSlots = 6;
+    static constexpr int kScratchSlots = 8;
+    static constexpr int kSlotGemmWorkspace = 7; // packed operands of split-BF16 GEMMs
 
     int tag = kTag;
     int device = 0;

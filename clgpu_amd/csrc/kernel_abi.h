@@ -38,20 +38,25 @@ enum { ICL_MATH_FP32 = 0, ICL_MATH_BF16X3 = 1 };
 
 /* One compiled GEMM implementation (a tile configuration with all its
  * op/edge instantiations). opa/opb: 0 = N, 1 = T, 2 = C. edge = 0 requires
- * M % bm == N % bn == K % bk == 0 and 16-byte aligned operand columns. */
+ * M % bm == N % bn == K % bk == 0 and 16-byte aligned operand columns.
+ * workspace (may be NULL): device bytes the launch needs in `ws` (e.g. for
+ * operands re-packed by a prepass); launch receives ws (NULL if none needed). */
 typedef struct icl_gemm_variant {
     const char* name;
     int kind;
     int math;
     int bm, bn, bk;
     int threads;
-    hipError_t (*launch)(const icl_gemm_args* args, int opa, int opb, int edge, hipStream_t stream);
+    hipError_t (*launch)(const icl_gemm_args* args, int opa, int opb, int edge, void* ws, hipStream_t stream);
+    size_t (*workspace)(const icl_gemm_args* args);
 } icl_gemm_variant;
 
-const icl_gemm_variant* icl_sgemm_f32_variants(int* count);
-const icl_gemm_variant* icl_sgemm_bf16x3_variants(int* count);
-const icl_gemm_variant* icl_cgemm_f32_variants(int* count);
-const icl_gemm_variant* icl_cgemm_bf16x3_variants(int* count);
+/* Registered GEMM implementations. Kernel translation units register their
+ * tables at library load (icl_register_gemm_variants); the host layer reads
+ * them back with icl_gemm_variant_count / icl_gemm_variant_at, ordered by name. */
+void icl_register_gemm_variants(const icl_gemm_variant* table, int count);
+int icl_gemm_variant_count(void);
+const icl_gemm_variant* icl_gemm_variant_at(int index);
 
 /* Triangular solve of one diagonal block (at most 64 x 64) in canonical
  * forward-substitution form L * X = s_pre * X0 (L lower triangular), with

@@ -1,0 +1,392 @@
+// Split-BF16 (x3) SGEMM on pre-split, pre-packed operands.
+//
+// A prepass (split_pack_kernel) reads op(A) (m x k) and op(B)^T (n x k) once,
+// splits every FP32 x into hi = bf16(x), lo = bf16(x - hi), and writes them in
+// a slab-major layout that is exactly the LDS image the GEMM wants:
+//
+//   P[ks][r][h][kk]   ks = k / 32, r = row (m for A, n for B), h = 0 hi / 1 lo,
+//                     kk = k % 32;  offset ((ks * Rpad + r) * 2 + h) * 32 + kk
+//
+// Rows are padded to the GEMM tile and K to 32 with zeros, so the GEMM main loop
+// has no bounds checks, no transposes and no conversions: each 32-deep slab of a
+// BM-row tile is one contiguous BM * 128-byte block, copied with 16-byte loads
+// into LDS rows of 64 bf16 (+16 bytes of padding), from which every lane reads
+// its 8-k fragments with 16-byte LDS loads for v_mfma_f32_32x32x16_bf16.
+// acc += Bh*Ah + Bl*Ah + Bh*Al in FP32, as in the unpacked kernel.
+#include "gemm_variants.hpp"
+
+namespace icl {
+
+constexpr int kPackK = 32;   // k per slab
+constexpr int kPackRow = 72; // LDS row: 32 hi + 32 lo + 8 pad bf16 = 144 bytes
+
+// ---------------------------------------------------------------------------
+// Prepass: 64 rows x 32 k per 256-thread block, transposed through LDS when
+// the rows are the contiguous dimension of the source.
+// element (r, k) of the source is X[(k) + r * ld] when KCONTIG, else X[r + k * ld].
+// ---------------------------------------------------------------------------
+template <bool KCONTIG>
+__global__ void __launch_bounds__(256)
+split_pack_kernel(const float* __restrict__ X, long ld, int R, int K, int Rpad, __bf16* __restrict__ P)
+{
+    __shared__ float T[64][kPackK + 1];
+    const int r0 = blockIdx.x * 64;
+    const int ks = blockIdx.y;
+    const int k0 = ks * kPackK;
+    const int t = threadIdx.x;
+    if constexpr (!KCONTIG) {
+        const int r = t & 63, kb = t >> 6;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = kb + 4 * i;
+            T[r][k] = (r0 + r < R && k0 + k < K) ? X[(long)(k0 + k) * ld + r0 + r] : 0.0f;
+        }
+    } else {
+        const int k = t & 31, rb = t >> 5;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int r = rb + 8 * i;
+            T[r][k] = (r0 + r < R && k0 + k < K) ? X[(long)(r0 + r) * ld + k0 + k] : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int r = t >> 2, kq = (t & 3) * 8;
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        __bf16 h, l;
+        split_bf16(T[r][kq + i], h, l);
+        hi[i] = h;
+        lo[i] = l;
+    }
+    __bf16* dst = P + ((size_t)ks * Rpad + r0 + r) * (2 * kPackK) + kq;
+    *reinterpret_cast<bf16x8*>(dst) = hi;
+    *reinterpret_cast<bf16x8*>(dst + kPackK) = lo;
+}
+
+struct PackedArgs {
+    icl_gemm_args g; // M, N, C, ldc, alpha, beta, mask (A/B/K fields unused here)
+    const __bf16* Ap;
+    const __bf16* Bp;
+    int Mpad, Npad, slabs;
+};
+
+// ---------------------------------------------------------------------------
+// GEMM on packed operands.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WVM, int WVN, bool EDGE>
+struct PackedCfg {
+    static constexpr int NT = WVM * WVN * 64;
+    static constexpr int WM = BM / WVM, WN = BN / WVN;
+    static constexpr int FM = WM / 32, FN = WN / 32;
+    static constexpr int A_ELEMS = BM * kPackRow, B_ELEMS = BN * kPackRow;
+    static constexpr int STAGE = A_ELEMS + B_ELEMS;
+    static constexpr int CA = BM * 8, CB = BN * 8; // 16-byte chunks per slab
+    static constexpr int PA = CA / NT, PB = CB / NT;
+    static constexpr size_t LDS_BYTES = 2 * (size_t)STAGE * sizeof(__bf16);
+    static_assert(CA % NT == 0 && CB % NT == 0, "slab chunks must divide among threads");
+    static_assert(WM % 32 == 0 && WN % 32 == 0, "bad wave tile");
+};
+
+template <int BM, int BN, int WVM, int WVN, bool EDGE>
+__global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_packed_kernel(PackedArgs p)
+{
+    using Cfg = PackedCfg<BM, BN, WVM, WVN, EDGE>;
+    extern __shared__ __attribute__((aligned(16))) __bf16 ldsb[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave % WVM) * Cfg::WM;
+    const int wn0 = (wave / WVM) * Cfg::WN;
+    int tm, tn;
+    tile_coords<8>(p.Mpad / BM, p.Npad / BN, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+
+    const size_t a_slab = (size_t)p.Mpad * (2 * kPackK); // elements per A slab
+    const size_t b_slab = (size_t)p.Npad * (2 * kPackK);
+    const __bf16* a_src = p.Ap + (size_t)m0 * (2 * kPackK);
+    const __bf16* b_src = p.Bp + (size_t)n0 * (2 * kPackK);
+
+    u32x4 ra[Cfg::PA], rb[Cfg::PB];
+    auto load = [&](int ks) {
+        const __bf16* a = a_src + ks * a_slab;
+        const __bf16* b = b_src + ks * b_slab;
+#pragma unroll
+        for (int i = 0; i < Cfg::PA; ++i) ra[i] = *reinterpret_cast<const u32x4*>(a + (size_t)(tid + Cfg::NT * i) * 8);
+#pragma unroll
+        for (int i = 0; i < Cfg::PB; ++i) rb[i] = *reinterpret_cast<const u32x4*>(b + (size_t)(tid + Cfg::NT * i) * 8);
+    };
+    auto store = [&](int s) {
+        __bf16* As = ldsb + s * Cfg::STAGE;
+        __bf16* Bs = As + Cfg::A_ELEMS;
+#pragma unroll
+        for (int i = 0; i < Cfg::PA; ++i) {
+            const int q = tid + Cfg::NT * i;
+            *reinterpret_cast<u32x4*>(As + (q >> 3) * kPackRow + (q & 7) * 8) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < Cfg::PB; ++i) {
+            const int q = tid + Cfg::NT * i;
+            *reinterpret_cast<u32x4*>(Bs + (q >> 3) * kPackRow + (q & 7) * 8) = rb[i];
+        }
+    };
+
+    f32x16 acc[Cfg::FM][Cfg::FN];
+#pragma unroll
+    for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = f32x16{};
+
+    const int S = p.slabs;
+    if (S > 0) {
+        load(0);
+        store(0);
+        __syncthreads();
+        for (int ks = 0; ks < S; ++ks) {
+            const int cur = ks & 1;
+            if (ks + 1 < S) load(ks + 1);
+            const __bf16* As = ldsb + cur * Cfg::STAGE;
+            const __bf16* Bs = As + Cfg::A_ELEMS;
+#pragma unroll
+            for (int kk = 0; kk < kPackK; kk += 16) {
+                const int kc = kk + 8 * (lane >> 5);
+                bf16x8 ah[Cfg::FM], al[Cfg::FM], bh[Cfg::FN], bl[Cfg::FN];
+#pragma unroll
+                for (int i = 0; i < Cfg::FM; ++i) {
+                    const __bf16* row = As + (wm0 + 32 * i + (lane & 31)) * kPackRow + kc;
+                    ah[i] = *reinterpret_cast<const bf16x8*>(row);
+                    al[i] = *reinterpret_cast<const bf16x8*>(row + kPackK);
+                }
+#pragma unroll
+                for (int j = 0; j < Cfg::FN; ++j) {
+                    const __bf16* row = Bs + (wn0 + 32 * j + (lane & 31)) * kPackRow + kc;
+                    bh[j] = *reinterpret_cast<const bf16x8*>(row);
+                    bl[j] = *reinterpret_cast<const bf16x8*>(row + kPackK);
+                }
+#pragma unroll
+                for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x16_bf16(bh[j], ah[i], acc[i][j]);
+#pragma unroll
+                for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x16_bf16(bl[j], ah[i], acc[i][j]);
+#pragma unroll
+                for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x16_bf16(bh[j], al[i], acc[i][j]);
+            }
+            if (ks + 1 < S) store(cur ^ 1);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < Cfg::FN; ++j)
+            store_real_tile<EDGE>(p.g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Same GEMM with the slab copies done by the LDS DMA path (global_load_lds_dwordx4):
+// the next slab streams straight into the other LDS stage while the current one
+// is multiplied, without passing through VGPRs.
+// One DMA instruction writes 64 lanes x 16 bytes = 1 KB contiguous LDS (8 rows of
+// 128 bytes), so rows are unpadded and bank conflicts on the 16-byte fragment
+// reads are avoided by storing logical 16-byte chunk c of row r at physical
+// chunk c ^ (r & 7).
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(1))) void global_void;
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int BM, int BN, int WVM, int WVN>
+struct DmaCfg {
+    static constexpr int NW = WVM * WVN;
+    static constexpr int NT = NW * 64;
+    static constexpr int WM = BM / WVM, WN = BN / WVN;
+    static constexpr int FM = WM / 32, FN = WN / 32;
+    static constexpr int ROWB = 2 * kPackK * 2;            // 128 bytes per LDS row
+    static constexpr int STAGE_BYTES = (BM + BN) * ROWB;
+    static constexpr int IA = BM / 8 / NW, IB = BN / 8 / NW; // DMA instructions per wave per slab
+    static constexpr size_t LDS_BYTES = 2 * (size_t)STAGE_BYTES;
+    static_assert((BM / 8) % NW == 0 && (BN / 8) % NW == 0, "rows must divide among waves");
+    static_assert(WM % 32 == 0 && WN % 32 == 0, "bad wave tile");
+};
+
+template <int BM, int BN, int WVM, int WVN, bool EDGE>
+__global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_dma_kernel(PackedArgs p)
+{
+    using Cfg = DmaCfg<BM, BN, WVM, WVN>;
+    extern __shared__ __attribute__((aligned(1024))) unsigned char ldsd[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm0 = (wave % WVM) * Cfg::WM;
+    const int wn0 = (wave / WVM) * Cfg::WN;
+    int tm, tn;
+    tile_coords<8>(p.Mpad / BM, p.Npad / BN, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+
+    const size_t a_slab = (size_t)p.Mpad * (2 * kPackK);
+    const size_t b_slab = (size_t)p.Npad * (2 * kPackK);
+    const __bf16* a_src = p.Ap + (size_t)m0 * (2 * kPackK);
+    const __bf16* b_src = p.Bp + (size_t)n0 * (2 * kPackK);
+    // lane -> (row within an 8-row DMA block, logical chunk)
+    const int lrow = lane >> 3;
+    const int lchunk = (lane & 7) ^ lrow;
+
+    auto issue = [&](int ks, int s) {
+        unsigned char* stage = ldsd + s * Cfg::STAGE_BYTES;
+        const __bf16* a = a_src + ks * a_slab;
+        const __bf16* b = b_src + ks * b_slab;
+#pragma unroll
+        for (int i = 0; i < Cfg::IA; ++i) {
+            const int r0 = 8 * (wave + Cfg::NW * i);
+            const __bf16* g = a + (size_t)(r0 + lrow) * (2 * kPackK) + lchunk * 8;
+            __builtin_amdgcn_global_load_lds((global_void*)g, (lds_void*)(stage + r0 * Cfg::ROWB), 16, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < Cfg::IB; ++i) {
+            const int r0 = 8 * (wave + Cfg::NW * i);
+            const __bf16* g = b + (size_t)(r0 + lrow) * (2 * kPackK) + lchunk * 8;
+            __builtin_amdgcn_global_load_lds((global_void*)g, (lds_void*)(stage + (BM + r0) * Cfg::ROWB), 16, 0, 0);
+        }
+    };
+
+    f32x16 acc[Cfg::FM][Cfg::FN];
+#pragma unroll
+    for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = f32x16{};
+
+    const int S = p.slabs;
+    if (S > 0) {
+        issue(0, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (int ks = 0; ks < S; ++ks) {
+            const int cur = ks & 1;
+            if (ks + 1 < S) issue(ks + 1, cur ^ 1);
+            const unsigned char* As = ldsd + cur * Cfg::STAGE_BYTES;
+            const unsigned char* Bs = As + BM * Cfg::ROWB;
+#pragma unroll
+            for (int kk = 0; kk < kPackK; kk += 16) {
+                const int c = (kk >> 3) + (lane >> 5); // logical hi chunk 0..3; lo = c + 4
+                bf16x8 ah[Cfg::FM], al[Cfg::FM], bh[Cfg::FN], bl[Cfg::FN];
+#pragma unroll
+                for (int i = 0; i < Cfg::FM; ++i) {
+                    const int r = wm0 + 32 * i + (lane & 31);
+                    const unsigned char* row = As + r * Cfg::ROWB;
+                    ah[i] = *reinterpret_cast<const bf16x8*>(row + ((c ^ (r & 7)) << 4));
+                    al[i] = *reinterpret_cast<const bf16x8*>(row + (((c + 4) ^ (r & 7)) << 4));
+                }
+#pragma unroll
+                for (int j = 0; j < Cfg::FN; ++j) {
+                    const int r = wn0 + 32 * j + (lane & 31);
+                    const unsigned char* row = Bs + r * Cfg::ROWB;
+                    bh[j] = *reinterpret_cast<const bf16x8*>(row + ((c ^ (r & 7)) << 4));
+                    bl[j] = *reinterpret_cast<const bf16x8*>(row + (((c + 4) ^ (r & 7)) << 4));
+                }
+#pragma unroll
+                for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x16_bf16(bh[j], ah[i], acc[i][j]);
+#pragma unroll
+                for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x16_bf16(bl[j], ah[i], acc[i][j]);
+#pragma unroll
+                for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x16_bf16(bh[j], al[i], acc[i][j]);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < Cfg::FN; ++j)
+            store_real_tile<EDGE>(p.g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
+}
+
+namespace {
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+template <int BM, int BN>
+size_t packed_workspace(const icl_gemm_args* g)
+{
+    const size_t kpad = (size_t)round_up(g->K, kPackK);
+    return ((size_t)round_up(g->M, BM) + (size_t)round_up(g->N, BN)) * kpad * 2 * sizeof(__bf16);
+}
+
+hipError_t pack(const float* X, long ld, int R, int K, int Rpad, int kcontig, __bf16* P, hipStream_t s)
+{
+    const dim3 grid((unsigned)(Rpad / 64), (unsigned)(round_up(K, kPackK) / kPackK));
+    if (kcontig) hipLaunchKernelGGL(split_pack_kernel<true>, grid, dim3(256), 0, s, X, ld, R, K, Rpad, P);
+    else hipLaunchKernelGGL(split_pack_kernel<false>, grid, dim3(256), 0, s, X, ld, R, K, Rpad, P);
+    return hipGetLastError();
+}
+
+template <int BM, int BN, int WVM, int WVN, bool DMA>
+hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, void* ws, hipStream_t s)
+{
+    if (!ws) return hipErrorInvalidValue;
+    PackedArgs p{};
+    p.g = *g;
+    p.Mpad = round_up(g->M, BM);
+    p.Npad = round_up(g->N, BN);
+    p.slabs = round_up(g->K, kPackK) / kPackK;
+    __bf16* Ap = static_cast<__bf16*>(ws);
+    __bf16* Bp = Ap + (size_t)p.Mpad * p.slabs * (2 * kPackK);
+    p.Ap = Ap;
+    p.Bp = Bp;
+    // op(A)(m, k): N -> A[m + k*lda] (rows contiguous), T/C -> A[k + m*lda] (k contiguous)
+    hipError_t e = pack(g->A, g->lda, g->M, g->K, p.Mpad, opa != 0, Ap, s);
+    if (e != hipSuccess) return e;
+    // op(B)(k, n) as rows n: N -> B[k + n*ldb] (k contiguous), T/C -> B[n + k*ldb]
+    e = pack(g->B, g->ldb, g->N, g->K, p.Npad, opb == 0, Bp, s);
+    if (e != hipSuccess) return e;
+    const bool even = g->M % BM == 0 && g->N % BN == 0;
+    (void)edge;
+    // launch_gemm sizes the grid from (M, N); the packed grid is the same.
+    if constexpr (DMA) {
+        using C = DmaCfg<BM, BN, WVM, WVN>;
+        if (even)
+            return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, g, s, p);
+        return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true>, g, s, p);
+    } else {
+        using C0 = PackedCfg<BM, BN, WVM, WVN, false>;
+        using C1 = PackedCfg<BM, BN, WVM, WVN, true>;
+        if (even)
+            return launch_gemm<BM, BN, C0::NT, C0::LDS_BYTES>(sgemm_bf16x3_packed_kernel<BM, BN, WVM, WVN, false>, g,
+                                                              s, p);
+        return launch_gemm<BM, BN, C1::NT, C1::LDS_BYTES>(sgemm_bf16x3_packed_kernel<BM, BN, WVM, WVN, true>, g, s,
+                                                          p);
+    }
+}
+
+} // namespace
+} // namespace icl
+
+#define ICL_SGEMM_BF16X3_PACKED(name, BM, BN, WVM, WVN, DMA)                                                           \
+    {name,                                                                                                             \
+     ICL_KIND_SGEMM,                                                                                                   \
+     ICL_MATH_BF16X3,                                                                                                  \
+     BM,                                                                                                               \
+     BN,                                                                                                               \
+     icl::kPackK,                                                                                                      \
+     64 * WVM * WVN,                                                                                                   \
+     icl::packed_launch<BM, BN, WVM, WVN, DMA>,                                                                        \
+     icl::packed_workspace<BM, BN>}
+
+static const icl_gemm_variant kTable[] = {
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3p_mfma_256x256w8", 256, 256, 4, 2, false),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3p_mfma_256x256w8n", 256, 256, 2, 4, false),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3p_mfma_256x256", 256, 256, 2, 2, false),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3p_mfma_128x128", 128, 128, 2, 2, false),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8n", 256, 256, 2, 4, true),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8", 256, 256, 4, 2, true),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256", 256, 256, 2, 2, true),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, true),
+};
+ICL_REGISTER_VARIANTS(kTable)

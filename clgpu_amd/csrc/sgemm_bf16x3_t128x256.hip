@@ -1,0 +1,7 @@
+// GEMM variant table (see gemm_variants.hpp).
+#include "gemm_variants.hpp"
+
+static const icl_gemm_variant kTable[] = {
+    ICL_SGEMM_BF16X3("sgemm_bf16x3_mfma_128x256", 128, 256, 16, 2, 2),
+};
+ICL_REGISTER_VARIANTS(kTable)

@@ -1,0 +1,7 @@
+// GEMM variant table (see gemm_variants.hpp).
+#include "gemm_variants.hpp"
+
+static const icl_gemm_variant kTable[] = {
+    ICL_SGEMM_F32("sgemm_f32_mfma_256x256k16", 256, 256, 16, 2, 2),
+};
+ICL_REGISTER_VARIANTS(kTable)

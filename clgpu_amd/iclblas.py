@@ -83,7 +83,15 @@ _SIGS = {
     "iclblasSynchronize": [_P],
     "iclblasSetImplementation": [_P, ct.c_char_p],
     "iclblasGetLastImplementation": [_P, ct.c_char_p, _I],
+    "iclblasGetImplementations": [ct.c_char_p, _I],
 }
+
+
+def implementation_names() -> list[str]:
+    """Names of every registered GEMM implementation (iclblasGetImplementations)."""
+    buf = ct.create_string_buffer(1 << 14)
+    check(load().iclblasGetImplementations(buf, len(buf)), "iclblasGetImplementations")
+    return buf.value.decode().split("\n") if buf.value else []
 
 
 def load() -> ct.CDLL:
@@ -92,6 +100,15 @@ def load() -> ct.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is not built; run `python -m clgpu_amd.build`")
+        # PyTorch-ROCm ships its own libamdhip64.so.7 with the same SONAME as the
+        # system ROCm runtime the library links. Whichever is loaded first serves the
+        # whole process; loading torch first keeps one HIP runtime that both torch and
+        # the library use (torch cannot initialise the GPU on a runtime it was not
+        # built with).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ct.CDLL(str(LIB_PATH))
         for name, args in _SIGS.items():
             f = getattr(L, name)

@@ -18,7 +18,7 @@ from clgpu_amd import iclblas as ib
 pytestmark = pytest.mark.gpu
 KATS = json.loads((Path(__file__).parent / "golden" / "kats.json").read_text())
 MODES = [ib.MATH_FP32, ib.MATH_SPLIT_BF16X3]
-IMPLS = ["cgemm_f32_mfma_64x64", "cgemm_f32_mfma_128x128", "cgemm_bf16x3_mfma_64x64", "cgemm_bf16x3_mfma_128x128"]
+IMPLS = [n for n in ib.implementation_names() if n.startswith("cgemm")]
 
 
 @pytest.fixture
@@ -102,7 +102,7 @@ def test_cgemm_error_bound(h, mode, ta, tb, shape):
     assert h.last_implementation().startswith("cgemm_bf16x3" if split else "cgemm_f32")
 
 
-@pytest.mark.parametrize("impl", IMPLS[2:])
+@pytest.mark.parametrize("impl", [n for n in IMPLS if "bf16x3" in n])
 def test_cgemm_split_forced_small_k_bound(h, impl):
     m, n, k = 40, 24, 20
     A = datagen.complex_matrix(m, k, name="A", seed=33)

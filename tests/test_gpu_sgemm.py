@@ -17,8 +17,14 @@ from clgpu_amd import iclblas as ib
 pytestmark = pytest.mark.gpu
 KATS = json.loads((Path(__file__).parent / "golden" / "kats.json").read_text())
 MODES = [ib.MATH_FP32, ib.MATH_SPLIT_BF16X3]
-F32_IMPLS = ["sgemm_f32_mfma_128x128", "sgemm_f32_mfma_256x256", "sgemm_f32_mfma_256x256w8"]
-BF16_IMPLS = ["sgemm_bf16x3_mfma_128x128", "sgemm_bf16x3_mfma_256x128"]
+
+def _registered(prefix):
+    from clgpu_amd import iclblas
+    return [n for n in iclblas.implementation_names() if n.startswith(prefix)]
+
+
+F32_IMPLS = _registered("sgemm_f32")
+BF16_IMPLS = _registered("sgemm_bf16x3")
 
 
 @pytest.fixture
