@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--impl", default="", help="comma list, one implementation name per path (empty = automatic)")
     ap.add_argument("--trans", default="NN", help="NN, NT, TN or TT for the timed op")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check (profiling runs)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the CGEMM / STRSM rows")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy kernel first (scales HBM byte counters)")
     return ap.parse_args()
@@ -71,6 +72,98 @@ def cpu_baseline(A: np.ndarray, B: np.ndarray, n: int, target_s: float) -> dict:
             "sample": f"SGEMM NN {m}x{cols}x{k} column slab of the {m}x{n}x{k} workload "
                       f"({cols} of {n} output columns, {dt:.1f} s), naive cpu_gemm loop order with a double "
                       f"accumulator, columns split over {threads} threads; rate extrapolates linearly in n"}
+
+
+def _time(stream, fn, steps: int, warmup: int) -> float:
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def extra_workloads(h, ib, stream, steps: int) -> dict:
+    """The other level-3 rows at their configured shapes: CGEMM 4096^3 NN and CN
+    (alpha = (1, 0.5), beta = (0.25, 0)) on both paths, and STRSM left/lower/N/
+    non-unit 4096 x 1024 plain and with 3 refinement sweeps. Errors against FP64
+    products / solves computed with torch on the GPU."""
+    import torch
+
+    from clgpu_amd import datagen
+    out = {}
+    dev = torch.device("cuda:0")
+    n = 4096
+    A = torch.from_numpy(np.ascontiguousarray(datagen.complex_matrix(n, n, name="A").T)).to(dev)
+    B = torch.from_numpy(np.ascontiguousarray(datagen.complex_matrix(n, n, name="B").T)).to(dev)
+    C0 = torch.from_numpy(np.ascontiguousarray(datagen.complex_matrix(n, n, name="C").T)).to(dev)
+    alpha, beta = 1 + 0.5j, 0.25
+    flops = 8.0 * n ** 3
+    cg = {}
+    for opname, ta in (("NN", 0), ("CN", 2)):
+        A64 = A.to(torch.complex128)
+        opAT = A64 if ta == 0 else A64.conj().t()  # storage of op(A)^T
+        ref = alpha * (B.to(torch.complex128) @ opAT) + beta * C0.to(torch.complex128)
+        absab = (B.to(torch.complex128).abs() @ opAT.abs())  # |z| bound, a little loose vs |re|+|im|
+        res = {}
+        for path, mode in (("fp32", ib.MATH_FP32), ("bf16x3", ib.MATH_SPLIT_BF16X3)):
+            h.set_math_mode(mode)
+            h.set_implementation(None)
+            C = C0.clone()
+
+            def step():
+                ib.check(ib.iclblasCgemm(h.value, ta, 0, n, n, n, alpha, A.data_ptr(), n, B.data_ptr(), n, beta,
+                                         C.data_ptr(), n), "iclblasCgemm")
+            step()
+            torch.cuda.synchronize()
+            err = (C.to(torch.complex128) - ref).abs()
+            ratio = float((err / (n * 2.0 ** -24 * (abs(alpha) * 2 * absab + abs(beta) * C0.abs().double()))).max())
+            Cw = C0.clone()
+            ms = _time(stream, lambda: ib.check(ib.iclblasCgemm(h.value, ta, 0, n, n, n, alpha, A.data_ptr(), n,
+                                                                B.data_ptr(), n, beta, Cw.data_ptr(), n)), steps, 2)
+            tf = flops / (ms * 1e-3) / 1e12
+            res[path] = {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
+                         "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA,
+                         "executed_frac_of_mfma_peak": (tf / PEAK_FP32_MFMA) if path == "fp32"
+                         else 3 * tf / PEAK_BF16_MFMA,
+                         "max_abs_err": float(err.max()), "normwise_rel_err": float(err.norm() / ref.abs().norm()),
+                         "max_err_over_bound_unit": ratio}
+            del err, C, Cw
+        cg[opname] = res
+        del ref, absab, A64, opAT
+    out["cgemm_4096"] = cg
+    h.set_math_mode(ib.MATH_FP32)
+    del A, B, C0
+    torch.cuda.empty_cache()
+
+    m, nr = 4096, 1024
+    T = datagen.triangular(m, lower=True, seed=0, poison=True)
+    Bm = datagen.matrix(m, nr, name="B", seed=0)
+    tT = torch.from_numpy(np.ascontiguousarray(T.T)).to(dev)
+    tB = torch.from_numpy(np.ascontiguousarray(Bm.T)).to(dev)
+    L64 = torch.nan_to_num(tT.double().t()).tril()
+    X64 = torch.linalg.solve_triangular(L64, tB.double().t(), upper=False)
+    flops_t = float(m) * m * nr
+    ts = {}
+    for label, sweeps in (("plain", 0), ("refined_3_sweeps", 3)):
+        X = tB.clone()
+        ib.check(ib.iclblasStrsmRefined(h.value, 0, 1, 0, 0, m, nr, 1.0, tT.data_ptr(), m, X.data_ptr(), m, sweeps))
+        torch.cuda.synchronize()
+        fe = float((X.double().t() - X64).norm() / X64.norm())
+        Xw = tB.clone()
+        ms = _time(stream, lambda: ib.check(ib.iclblasStrsmRefined(h.value, 0, 1, 0, 0, m, nr, 1.0, tT.data_ptr(), m,
+                                                                   Xw.data_ptr(), m, sweeps)), max(2, steps // 4), 1)
+        tf = flops_t / (ms * 1e-3) / 1e12
+        ts[label] = {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
+                     "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA, "forward_rel_err_fro": fe}
+    out["strsm_4096x1024"] = ts
+    return out
 
 
 def main():
@@ -201,6 +294,11 @@ def main():
                                  "kernel": r["implementation"]}
         if "bf16x3" in results:
             line["split_over_fp32"] = results["bf16x3"]["tflops"] / r["tflops"]
+    if not a.no_extra:
+        print("[bench] CGEMM / STRSM rows", file=sys.stderr, flush=True)
+        del C64, absAB
+        torch.cuda.empty_cache()
+        line["other_rows"] = extra_workloads(h, ib, stream, a.steps)
     if not a.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(A, B, n, a.cpu_seconds)

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 
@@ -37,23 +38,27 @@ bool gemm_needs_edge(const icl_gemm_variant& v, const GemmProblem& p)
     return !(even && aligned);
 }
 
-// Score model: the fraction of the last dispatch wave that is filled, times the
-// tile's relative MFMA efficiency (bigger wave tiles re-use each LDS fragment more),
-// with a small bonus for problems that can use the unchecked instantiation.
+// Score = 1 / estimated run time (seconds). Each variant carries its measured
+// full-GPU rate R (TFLOP/s at 8192^3) and occupancy occ (workgroups per CU).
+// With 256 CUs, 256*occ tiles run concurrently, each tile taking
+// t_tile = 2*bm*bn*K*flops_per_mac_scale / (R / (256*occ)); the grid runs in
+// ceil(tiles / (256*occ)) such rounds, so small or ragged problems pay for idle
+// CUs and padded tiles. Variants with a pack prepass add its HBM time
+// ((M + N) * K * (4 read + 4 written) bytes at 5 TB/s). The bounds-checked
+// instantiation is taken to run 10% slower.
 float gemm_score(const icl_gemm_variant& v, const GemmProblem& p)
 {
     const icl_gemm_args& g = *p.args;
     constexpr double kCUs = 256.0;
+    const double macs = p.kind == ICL_KIND_CGEMM ? 4.0 : 1.0; // real MACs per element MAC
+    const double conc = kCUs * (v.occupancy > 0 ? v.occupancy : 1);
     const double tiles = double((g.M + v.bm - 1) / v.bm) * double((g.N + v.bn - 1) / v.bn);
-    const double waves = tiles / kCUs;
-    const double fill = waves >= 1.0 ? waves / std::ceil(waves) : waves;
-    const double tile_eff = double(v.bm * v.bn) / (256.0 * 256.0);
-    const double eff = 0.75 + 0.25 * tile_eff;
-    // useful work fraction of the padded tiles
-    const double useful = double(g.M) * double(g.N) / (tiles * v.bm * v.bn);
-    double s = fill * eff * useful;
-    if (!gemm_needs_edge(v, p)) s *= 1.05;
-    return (float)s;
+    const double rate = (v.tflops_hint > 0 ? v.tflops_hint : 1.0) * 1e12;
+    const double t_tile = 2.0 * macs * v.bm * v.bn * double(g.K) * conc / rate;
+    double t = std::ceil(tiles / conc) * t_tile;
+    if (gemm_needs_edge(v, p)) t *= 1.10;
+    if (v.workspace) t += (double(g.M) + double(g.N)) * double(g.K) * 8.0 * macs / 5e12;
+    return (float)(1.0 / t);
 }
 
 static bool forced_is_gemm_of_kind(const std::string& name, int kind)
@@ -304,15 +309,50 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
     const bool lower_eff = (uplo == ICLBLAS_FILL_MODE_LOWER) != t.trans; // op(A) lower triangular
     t.fwd = t.left ? lower_eff : !lower_eff;
     const int order = t.left ? m : n;
+    const bool forced_rec = h->forced_impl == "strsm_recursive";
     if (order <= kLeaf) {
         // Single diagonal block: alpha applied to the solution, as the reference
         // kernel does (Strsm_naive.cl:35).
         t.solve(0, order, alpha);
-    } else {
+        if (record) h->last_impl = "strsm_diag64";
+    } else if (forced_rec) {
         t.solve(0, order, 1.0f);
         if (alpha != 1.0f) hip_check(icl_axpby_2d(m, n, alpha, B, ldb, 0.0f, B, ldb, h->stream), "axpby");
+        if (record) h->last_impl = "strsm_recursive";
+    } else {
+        // Canonical forward view of the whole system (as in Trsm::leaf with p0 = 0).
+        icl_trsm_blocked_args a{};
+        a.nsys = order;
+        a.nlines = t.left ? n : m;
+        a.unit = t.unit ? 1 : 0;
+        const long e = order - 1;
+        if (t.left) {
+            a.x_sj = ldb;
+            if (t.fwd) {
+                a.L = A, a.l_si = t.si, a.l_sk = t.sk, a.B = B, a.x_si = 1;
+            } else {
+                a.L = A + e * (t.si + t.sk), a.l_si = -t.si, a.l_sk = -t.sk, a.B = B + e, a.x_si = -1;
+            }
+        } else {
+            a.x_sj = 1;
+            if (t.fwd) {
+                a.L = A, a.l_si = t.sk, a.l_sk = t.si, a.B = B, a.x_si = ldb;
+            } else {
+                a.L = A + e * (t.si + t.sk), a.l_si = -t.sk, a.l_sk = -t.si, a.B = B + e * ldb, a.x_si = -ldb;
+            }
+        }
+        const int nblk = (order + kLeaf - 1) / kLeaf;
+        const size_t inv_elems = (size_t)nblk * kLeaf * kLeaf;
+        float* ws = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
+            (inv_elems + (size_t)order * a.nlines) * sizeof(float), h->stream));
+        a.invD = ws;
+        a.X = ws + inv_elems;
+        a.ldx = order;
+        hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
+        for (int b = 0; b < nblk; ++b) hip_check(icl_trsm_step(&a, b, h->stream), "trsm_step");
+        hip_check(icl_trsm_scatter(&a, alpha, h->stream), "trsm_scatter");
+        if (record) h->last_impl = "strsm_blocked_inv64";
     }
-    if (record) h->last_impl = "strsm_blocked_rec64";
 }
 
 void strsm_refined_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,

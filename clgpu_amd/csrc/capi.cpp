@@ -28,6 +28,11 @@ namespace {
 
 enum Slot { kSlotA = 0, kSlotB = 1, kSlotC = 2, kSlotW0 = 3, kSlotW1 = 4, kSlotW2 = 5 };
 
+// STRSM implementations (selected by size; forceable by name): the blocked solve
+// with inverted 64 x 64 diagonal blocks (default above 64 unknowns) and the
+// recursive split into diagonal-block solves plus GEMM updates.
+const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_recursive"};
+
 template <typename F>
 iclblasStatus_t guarded(F&& f)
 {
@@ -354,6 +359,11 @@ ICLBLAS_API iclblasStatus_t iclblasSetImplementation(iclblasHandle_t handle, con
             handle->forced_impl = name;
             return ICLBLAS_STATUS_SUCCESS;
         }
+    for (const char* t : kTrsmImpls)
+        if (std::strcmp(t, name) == 0) {
+            handle->forced_impl = name;
+            return ICLBLAS_STATUS_SUCCESS;
+        }
     return ICLBLAS_STATUS_INVALID_VALUE;
 }
 
@@ -364,6 +374,10 @@ ICLBLAS_API iclblasStatus_t iclblasGetImplementations(char* buf, int len)
     for (const icl_gemm_variant* v : icl::gemm_registry()) {
         if (!all.empty()) all += '\n';
         all += v->name;
+    }
+    for (const char* t : kTrsmImpls) {
+        all += '\n';
+        all += t;
     }
     if ((int)all.size() + 1 > len) return ICLBLAS_STATUS_INVALID_VALUE;
     std::memcpy(buf, all.c_str(), all.size() + 1);

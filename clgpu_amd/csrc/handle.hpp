@@ -73,6 +73,7 @@ private:
 struct iclblasContext {
     static constexpr int kTag = 0xB1A5;
     static constexpr int kScratchSlots = 8;
+    static constexpr int kSlotTrsmWorkspace = 6; // inverted diagonal blocks + solution of blocked STRSM
     static constexpr int kSlotGemmWorkspace = 7; // packed operands of split-BF16 GEMMs
 
     int tag = kTag;

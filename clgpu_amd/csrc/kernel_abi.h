@@ -49,6 +49,8 @@ typedef struct icl_gemm_variant {
     int threads;
     hipError_t (*launch)(const icl_gemm_args* args, int opa, int opb, int edge, void* ws, hipStream_t stream);
     size_t (*workspace)(const icl_gemm_args* args);
+    float tflops_hint; /* measured useful TFLOP/s on a full MI355X (m = n = k = 8192, NN) */
+    int occupancy;     /* workgroups resident per CU */
 } icl_gemm_variant;
 
 /* Registered GEMM implementations. Kernel translation units register their
@@ -75,6 +77,28 @@ typedef struct icl_trsm_diag_args {
     float s_pre, s_post;
 } icl_trsm_diag_args;
 hipError_t icl_trsm_diag(const icl_trsm_diag_args* a, hipStream_t stream);
+
+/* Blocked solve of the canonical lower-triangular system L X = B with nsys
+ * unknowns and nlines right-hand sides: L(i, k) = L[i * l_si + k * l_sk],
+ * B(i, j) = B[i * x_si + j * x_sj] (strides may be negative). The 64 x 64
+ * diagonal blocks are inverted into invD (ceil(nsys/64) * 4096 floats); step b
+ * forms X_b = inv(L_bb) B_b into the dense X (ld = ldx) and subtracts L_rb X_b
+ * from the trailing rows of B (B is used as scratch); scatter writes
+ * alpha * X back through the view of B. */
+typedef struct icl_trsm_blocked_args {
+    int nsys, nlines;
+    const float* L;
+    long l_si, l_sk;
+    float* B;
+    long x_si, x_sj;
+    float* X;
+    long ldx;
+    float* invD;
+    int unit;
+} icl_trsm_blocked_args;
+hipError_t icl_trsm_invert_blocks(const icl_trsm_blocked_args* a, hipStream_t stream);
+hipError_t icl_trsm_step(const icl_trsm_blocked_args* a, int block, hipStream_t stream);
+hipError_t icl_trsm_scatter(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream);
 
 /* R = fp32(alpha * B0 - P * Q) with the products and the sum in FP64.
  * P is M x K, Q is K x N (fp32, column-major, optionally transposed); the

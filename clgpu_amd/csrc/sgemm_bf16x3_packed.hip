@@ -9,16 +9,15 @@
 //
 // Rows are padded to the GEMM tile and K to 32 with zeros, so the GEMM main loop
 // has no bounds checks, no transposes and no conversions: each 32-deep slab of a
-// BM-row tile is one contiguous BM * 128-byte block, copied with 16-byte loads
-// into LDS rows of 64 bf16 (+16 bytes of padding), from which every lane reads
-// its 8-k fragments with 16-byte LDS loads for v_mfma_f32_32x32x16_bf16.
+// BM-row tile is one contiguous BM * 128-byte block, copied by the LDS DMA path
+// into LDS rows of 64 bf16, from which every lane reads its 8-k fragments with
+// 16-byte LDS loads for v_mfma_f32_32x32x16_bf16.
 // acc += Bh*Ah + Bl*Ah + Bh*Al in FP32, as in the unpacked kernel.
 #include "gemm_variants.hpp"
 
 namespace icl {
 
-constexpr int kPackK = 32;   // k per slab
-constexpr int kPackRow = 72; // LDS row: 32 hi + 32 lo + 8 pad bf16 = 144 bytes
+constexpr int kPackK = 32; // k per slab
 
 // ---------------------------------------------------------------------------
 // Prepass: 64 rows x 32 k per 256-thread block, transposed through LDS when
@@ -72,123 +71,9 @@ struct PackedArgs {
 };
 
 // ---------------------------------------------------------------------------
-// GEMM on packed operands.
-// ---------------------------------------------------------------------------
-template <int BM, int BN, int WVM, int WVN, bool EDGE>
-struct PackedCfg {
-    static constexpr int NT = WVM * WVN * 64;
-    static constexpr int WM = BM / WVM, WN = BN / WVN;
-    static constexpr int FM = WM / 32, FN = WN / 32;
-    static constexpr int A_ELEMS = BM * kPackRow, B_ELEMS = BN * kPackRow;
-    static constexpr int STAGE = A_ELEMS + B_ELEMS;
-    static constexpr int CA = BM * 8, CB = BN * 8; // 16-byte chunks per slab
-    static constexpr int PA = CA / NT, PB = CB / NT;
-    static constexpr size_t LDS_BYTES = 2 * (size_t)STAGE * sizeof(__bf16);
-    static_assert(CA % NT == 0 && CB % NT == 0, "slab chunks must divide among threads");
-    static_assert(WM % 32 == 0 && WN % 32 == 0, "bad wave tile");
-};
-
-template <int BM, int BN, int WVM, int WVN, bool EDGE>
-__global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_packed_kernel(PackedArgs p)
-{
-    using Cfg = PackedCfg<BM, BN, WVM, WVN, EDGE>;
-    extern __shared__ __attribute__((aligned(16))) __bf16 ldsb[];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm0 = (wave % WVM) * Cfg::WM;
-    const int wn0 = (wave / WVM) * Cfg::WN;
-    int tm, tn;
-    tile_coords<8>(p.Mpad / BM, p.Npad / BN, tm, tn);
-    const int m0 = tm * BM, n0 = tn * BN;
-
-    const size_t a_slab = (size_t)p.Mpad * (2 * kPackK); // elements per A slab
-    const size_t b_slab = (size_t)p.Npad * (2 * kPackK);
-    const __bf16* a_src = p.Ap + (size_t)m0 * (2 * kPackK);
-    const __bf16* b_src = p.Bp + (size_t)n0 * (2 * kPackK);
-
-    u32x4 ra[Cfg::PA], rb[Cfg::PB];
-    auto load = [&](int ks) {
-        const __bf16* a = a_src + ks * a_slab;
-        const __bf16* b = b_src + ks * b_slab;
-#pragma unroll
-        for (int i = 0; i < Cfg::PA; ++i) ra[i] = *reinterpret_cast<const u32x4*>(a + (size_t)(tid + Cfg::NT * i) * 8);
-#pragma unroll
-        for (int i = 0; i < Cfg::PB; ++i) rb[i] = *reinterpret_cast<const u32x4*>(b + (size_t)(tid + Cfg::NT * i) * 8);
-    };
-    auto store = [&](int s) {
-        __bf16* As = ldsb + s * Cfg::STAGE;
-        __bf16* Bs = As + Cfg::A_ELEMS;
-#pragma unroll
-        for (int i = 0; i < Cfg::PA; ++i) {
-            const int q = tid + Cfg::NT * i;
-            *reinterpret_cast<u32x4*>(As + (q >> 3) * kPackRow + (q & 7) * 8) = ra[i];
-        }
-#pragma unroll
-        for (int i = 0; i < Cfg::PB; ++i) {
-            const int q = tid + Cfg::NT * i;
-            *reinterpret_cast<u32x4*>(Bs + (q >> 3) * kPackRow + (q & 7) * 8) = rb[i];
-        }
-    };
-
-    f32x16 acc[Cfg::FM][Cfg::FN];
-#pragma unroll
-    for (int i = 0; i < Cfg::FM; ++i)
-#pragma unroll
-        for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = f32x16{};
-
-    const int S = p.slabs;
-    if (S > 0) {
-        load(0);
-        store(0);
-        __syncthreads();
-        for (int ks = 0; ks < S; ++ks) {
-            const int cur = ks & 1;
-            if (ks + 1 < S) load(ks + 1);
-            const __bf16* As = ldsb + cur * Cfg::STAGE;
-            const __bf16* Bs = As + Cfg::A_ELEMS;
-#pragma unroll
-            for (int kk = 0; kk < kPackK; kk += 16) {
-                const int kc = kk + 8 * (lane >> 5);
-                bf16x8 ah[Cfg::FM], al[Cfg::FM], bh[Cfg::FN], bl[Cfg::FN];
-#pragma unroll
-                for (int i = 0; i < Cfg::FM; ++i) {
-                    const __bf16* row = As + (wm0 + 32 * i + (lane & 31)) * kPackRow + kc;
-                    ah[i] = *reinterpret_cast<const bf16x8*>(row);
-                    al[i] = *reinterpret_cast<const bf16x8*>(row + kPackK);
-                }
-#pragma unroll
-                for (int j = 0; j < Cfg::FN; ++j) {
-                    const __bf16* row = Bs + (wn0 + 32 * j + (lane & 31)) * kPackRow + kc;
-                    bh[j] = *reinterpret_cast<const bf16x8*>(row);
-                    bl[j] = *reinterpret_cast<const bf16x8*>(row + kPackK);
-                }
-#pragma unroll
-                for (int i = 0; i < Cfg::FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x16_bf16(bh[j], ah[i], acc[i][j]);
-#pragma unroll
-                for (int i = 0; i < Cfg::FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x16_bf16(bl[j], ah[i], acc[i][j]);
-#pragma unroll
-                for (int i = 0; i < Cfg::FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x16_bf16(bh[j], al[i], acc[i][j]);
-            }
-            if (ks + 1 < S) store(cur ^ 1);
-            __syncthreads();
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < Cfg::FM; ++i)
-#pragma unroll
-        for (int j = 0; j < Cfg::FN; ++j)
-            store_real_tile<EDGE>(p.g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
-}
-
-// ---------------------------------------------------------------------------
-// Same GEMM with the slab copies done by the LDS DMA path (global_load_lds_dwordx4):
-// the next slab streams straight into the other LDS stage while the current one
-// is multiplied, without passing through VGPRs.
+// GEMM on the packed operands. Slab copies use the LDS DMA path
+// (global_load_lds_dwordx4): the next slab streams straight into the other LDS
+// stage while the current one is multiplied, without passing through VGPRs.
 // One DMA instruction writes 64 lanes x 16 bytes = 1 KB contiguous LDS (8 rows of
 // 128 bytes), so rows are unpadded and bank conflicts on the 16-byte fragment
 // reads are avoided by storing logical 16-byte chunk c of row r at physical
@@ -327,7 +212,7 @@ hipError_t pack(const float* X, long ld, int R, int K, int Rpad, int kcontig, __
     return hipGetLastError();
 }
 
-template <int BM, int BN, int WVM, int WVN, bool DMA>
+template <int BM, int BN, int WVM, int WVN>
 hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, void* ws, hipStream_t s)
 {
     if (!ws) return hipErrorInvalidValue;
@@ -349,26 +234,17 @@ hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, voi
     const bool even = g->M % BM == 0 && g->N % BN == 0;
     (void)edge;
     // launch_gemm sizes the grid from (M, N); the packed grid is the same.
-    if constexpr (DMA) {
-        using C = DmaCfg<BM, BN, WVM, WVN>;
-        if (even)
-            return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, g, s, p);
-        return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true>, g, s, p);
-    } else {
-        using C0 = PackedCfg<BM, BN, WVM, WVN, false>;
-        using C1 = PackedCfg<BM, BN, WVM, WVN, true>;
-        if (even)
-            return launch_gemm<BM, BN, C0::NT, C0::LDS_BYTES>(sgemm_bf16x3_packed_kernel<BM, BN, WVM, WVN, false>, g,
-                                                              s, p);
-        return launch_gemm<BM, BN, C1::NT, C1::LDS_BYTES>(sgemm_bf16x3_packed_kernel<BM, BN, WVM, WVN, true>, g, s,
-                                                          p);
-    }
+    using C = DmaCfg<BM, BN, WVM, WVN>;
+    if (even)
+        return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, g, s, p);
+    return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true>, g, s, p);
 }
 
 } // namespace
 } // namespace icl
 
-#define ICL_SGEMM_BF16X3_PACKED(name, BM, BN, WVM, WVN, DMA)                                                           \
+// TF: measured useful TFLOP/s at m = n = k = 8192 (NN), pack prepass included; OCC: workgroups per CU.
+#define ICL_SGEMM_BF16X3_PACKED(name, BM, BN, WVM, WVN, TF, OCC)                                                       \
     {name,                                                                                                             \
      ICL_KIND_SGEMM,                                                                                                   \
      ICL_MATH_BF16X3,                                                                                                  \
@@ -376,17 +252,14 @@ hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, voi
      BN,                                                                                                               \
      icl::kPackK,                                                                                                      \
      64 * WVM * WVN,                                                                                                   \
-     icl::packed_launch<BM, BN, WVM, WVN, DMA>,                                                                        \
-     icl::packed_workspace<BM, BN>}
+     icl::packed_launch<BM, BN, WVM, WVN>,                                                                             \
+     icl::packed_workspace<BM, BN>,                                                                                    \
+     TF,                                                                                                               \
+     OCC}
 
 static const icl_gemm_variant kTable[] = {
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3p_mfma_256x256w8", 256, 256, 4, 2, false),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3p_mfma_256x256w8n", 256, 256, 2, 4, false),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3p_mfma_256x256", 256, 256, 2, 2, false),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3p_mfma_128x128", 128, 128, 2, 2, false),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8n", 256, 256, 2, 4, true),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8", 256, 256, 4, 2, true),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256", 256, 256, 2, 2, true),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, true),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8", 256, 256, 4, 2, 405.8f, 1),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8n", 256, 256, 2, 4, 400.8f, 1),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, 344.6f, 2),
 };
 ICL_REGISTER_VARIANTS(kTable)

@@ -11,26 +11,52 @@ namespace {
 
 constexpr int kNB = 64; // largest diagonal block; one lane owns one right-hand side
 
-// One wavefront solves L * x = s_pre * b for 64 independent right-hand sides,
-// one per lane, with x held in 64 registers. L is staged in LDS transposed
-// (Lt[i][k] = L(k, i)) so that the update of pivot i reads a contiguous run of
-// the column below the diagonal; every lane reads the same address (broadcast).
-__global__ void __launch_bounds__(64) trsm_diag_kernel(icl_trsm_diag_args a)
-{
-    __shared__ float Lt[kNB][kNB + 4];
-    __shared__ float dinv_src[kNB];
-    const int lane = threadIdx.x;
-    const int nb = a.nb;
+// A lower-triangular diagonal block (nb <= 64) staged in LDS transposed,
+// Lt[i][k] = L(k, i) for k > i, so that the update of pivot i reads a contiguous
+// run of the column below the diagonal with every lane on the same address
+// (broadcast). Unused entries are 0 so the fully unrolled update never reads
+// garbage; diag[i] = L(i, i) (1 for a unit diagonal, which is never read).
+struct DiagBlock {
+    float Lt[kNB][kNB + 4];
+    float diag[kNB];
+};
 
-    // Stage L: column i of L (rows k > i) into Lt[i][k]; unused entries are 0 so
-    // that the fully unrolled update below never reads garbage.
+__device__ __forceinline__ void stage_diag_block(DiagBlock& s, const float* L, long l_si, long l_sk, int nb, int unit,
+                                                 int lane)
+{
     for (int i = 0; i < kNB; ++i) {
         const int k = lane;
         float v = 0.0f;
-        if (i < nb && k < nb && k > i) v = a.L[(long)k * a.l_si + (long)i * a.l_sk];
-        Lt[i][k] = v;
+        if (i < nb && k < nb && k > i) v = L[(long)k * l_si + (long)i * l_sk];
+        s.Lt[i][k] = v;
     }
-    if (lane < nb) dinv_src[lane] = a.unit ? 1.0f : a.L[(long)lane * a.l_si + (long)lane * a.l_sk];
+    if (lane < nb) s.diag[lane] = unit ? 1.0f : L[(long)lane * l_si + (long)lane * l_sk];
+}
+
+// Forward substitution L x = x in registers (the per-column recurrence of the
+// reference kernel, Strsm_naive.cl:28-37).
+__device__ __forceinline__ void solve_in_registers(float (&x)[kNB], const DiagBlock& s, int nb, int unit)
+{
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) {
+        if (i < nb) {
+            float xi = x[i];
+            if (!unit) xi = xi / s.diag[i];
+            x[i] = xi;
+#pragma unroll
+            for (int k = i + 1; k < kNB; ++k) x[k] -= s.Lt[i][k] * xi;
+        }
+    }
+}
+
+// One wavefront solves L * x = s_pre * b for 64 independent right-hand sides,
+// one per lane, with x held in 64 registers.
+__global__ void __launch_bounds__(64) trsm_diag_kernel(icl_trsm_diag_args a)
+{
+    __shared__ DiagBlock s;
+    const int lane = threadIdx.x;
+    const int nb = a.nb;
+    stage_diag_block(s, a.L, a.l_si, a.l_sk, nb, a.unit, lane);
     __syncthreads();
 
     const long j = (long)blockIdx.x * kNB + lane;
@@ -40,23 +66,34 @@ __global__ void __launch_bounds__(64) trsm_diag_kernel(icl_trsm_diag_args a)
     float x[kNB];
 #pragma unroll
     for (int i = 0; i < kNB; ++i) x[i] = (active && i < nb) ? a.s_pre * xcol[(long)i * a.x_si] : 0.0f;
-
-#pragma unroll
-    for (int i = 0; i < kNB; ++i) {
-        if (i < nb) {
-            float xi = x[i];
-            if (!a.unit) xi = xi / dinv_src[i];
-            x[i] = xi;
-#pragma unroll
-            for (int k = i + 1; k < kNB; ++k) x[k] -= Lt[i][k] * xi;
-        }
-    }
-
+    solve_in_registers(x, s, nb, a.unit);
     if (active) {
 #pragma unroll
         for (int i = 0; i < kNB; ++i)
             if (i < nb) xcol[(long)i * a.x_si] = a.s_post * x[i];
     }
+}
+
+// Inverts every 64 x 64 diagonal block of the canonical lower-triangular system
+// (workgroup b handles rows/columns [64 b, 64 b + nb)): lane j solves
+// L_bb y = e_j in registers, and column j of inv(L_bb) is written to
+// invD[b * 4096 + j * 64 + i] (column-major, rows and columns beyond nb zero).
+__global__ void __launch_bounds__(64) trsm_invert_kernel(icl_trsm_blocked_args a)
+{
+    __shared__ DiagBlock s;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    const int r0 = b * kNB;
+    const int nb = min(kNB, a.nsys - r0);
+    stage_diag_block(s, a.L + (long)r0 * (a.l_si + a.l_sk), a.l_si, a.l_sk, nb, a.unit, lane);
+    __syncthreads();
+    float x[kNB];
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) x[i] = (i == lane && lane < nb) ? 1.0f : 0.0f;
+    solve_in_registers(x, s, nb, a.unit);
+    float* col = a.invD + (long)b * kNB * kNB + (long)lane * kNB;
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) col[i] = (lane < nb && i < nb) ? x[i] : 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -152,6 +189,135 @@ __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
 }
 
 // ---------------------------------------------------------------------------
+// Blocked solve with inverted 64 x 64 diagonal blocks. Step b of the canonical
+// forward substitution, for every group of 64 right-hand sides (lines):
+//   X_b = inv(L_bb) * B_b                          (64 x 64 x 64 FP32 MFMA)
+//   B_r -= L_rb * X_b   for the trailing rows r    (128 rows per workgroup)
+// Each workgroup of the step recomputes X_b for its lines from B_b, which no
+// workgroup of the step writes, so the workgroups are independent; the one with
+// group 0 also stores X_b to the dense solution X. L, B may be strided either
+// way (canonical views of all side/uplo/trans variants).
+// ---------------------------------------------------------------------------
+constexpr int kStepCols = 64;  // lines per workgroup
+constexpr int kStepRows = 128; // trailing rows per workgroup
+constexpr int kLdD = kNB + 4, kLdB = kStepCols + 4, kLdL = kStepRows + 4;
+constexpr size_t kStepLds = sizeof(float) * ((size_t)kNB * kLdD + (size_t)kNB * kLdB + (size_t)kNB * kLdL);
+
+// T[k * LDT + r] = src[r * s_r + k * s_k] for r < R, k < KN (zero beyond rmax / kmax);
+// the thread-to-element map follows whichever of s_r, s_k is the unit stride.
+template <int KN, int R, int LDT>
+__device__ __forceinline__ void load_tile(float* T, const float* src, long s_r, long s_k, int rmax, int kmax, int tid)
+{
+    if (s_r == 1 || s_r == -1) {
+        for (int e = tid; e < KN * R; e += 256) {
+            const int r = e % R, k = e / R;
+            T[k * LDT + r] = (r < rmax && k < kmax) ? src[(long)r * s_r + (long)k * s_k] : 0.0f;
+        }
+    } else {
+        for (int e = tid; e < KN * R; e += 256) {
+            const int k = e % KN, r = e / KN;
+            T[k * LDT + r] = (r < rmax && k < kmax) ? src[(long)r * s_r + (long)k * s_k] : 0.0f;
+        }
+    }
+}
+
+// LINES_CONTIG: the lines of B are its contiguous dimension (right-side solves),
+// so the trailing update maps lanes to lines for coalesced read-modify-writes.
+template <bool LINES_CONTIG>
+__global__ void __launch_bounds__(256) trsm_step_kernel(icl_trsm_blocked_args a, int blk, int groups)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    float* D = sm;                 // D[k][r]  = inv(L_bb)(r, k)
+    float* BX = D + kNB * kLdD;    // BX[k][c] = B_b(k, c), then X_b(k, c)
+    float* Lr = BX + kNB * kLdB;   // Lr[k][r] = L(rb + r, r0 + k)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ncb = (a.nlines + kStepCols - 1) / kStepCols;
+    const int cb = blockIdx.x % ncb, g = blockIdx.x / ncb;
+    const int r0 = blk * kNB, c0 = cb * kStepCols;
+    const int rb = r0 + kNB + g * kStepRows;
+    const int ncols = min(kStepCols, a.nlines - c0);
+    const bool update = rb < a.nsys;
+
+    const float* inv = a.invD + (long)blk * kNB * kNB;
+    for (int e = tid; e < kNB * kNB; e += 256) D[(e / kNB) * kLdD + (e % kNB)] = inv[e];
+    load_tile<kNB, kStepCols, kLdB>(BX, a.B + (long)r0 * a.x_si + (long)c0 * a.x_sj, a.x_sj, a.x_si, ncols,
+                                    a.nsys - r0, tid);
+    if (update)
+        load_tile<kNB, kStepRows, kLdL>(Lr, a.L + (long)rb * a.l_si + (long)r0 * a.l_sk, a.l_si, a.l_sk,
+                                        a.nsys - rb, a.nsys - r0, tid);
+    __syncthreads();
+
+    // X_b quadrant of this wave: rows wr.., lines wc.. (lane <-> row)
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    f32x16 x = f32x16{};
+#pragma unroll 8
+    for (int kk = 0; kk < kNB; kk += 2) {
+        const int kr = kk + (lane >> 5);
+        x = icl::mfma_32x32x2_f32(BX[kr * kLdB + wc + (lane & 31)], D[kr * kLdD + wr + (lane & 31)], x);
+    }
+    __syncthreads(); // every wave is done reading B_b
+    {
+        const int row = wr + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int col = wc + icl::mfma32_row(lane, r);
+            BX[row * kLdB + col] = x[r];
+            if (g == 0 && r0 + row < a.nsys && col < ncols) a.X[(long)(r0 + row) + (long)(c0 + col) * a.ldx] = x[r];
+        }
+    }
+    if (!update) return;
+    __syncthreads();
+
+    // B(rb + row, c0 + col) -= sum_k L(rb + row, r0 + k) X_b(k, col); wave owns 32 rows, both line halves
+    f32x16 u[2] = {f32x16{}, f32x16{}};
+#pragma unroll 4
+    for (int kk = 0; kk < kNB; kk += 2) {
+        const int kr = kk + (lane >> 5);
+        const float l = Lr[kr * kLdL + 32 * wave + (lane & 31)];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const float xv = BX[kr * kLdB + 32 * j + (lane & 31)];
+            u[j] = LINES_CONTIG ? icl::mfma_32x32x2_f32(l, xv, u[j]) : icl::mfma_32x32x2_f32(xv, l, u[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            int row, col;
+            if (LINES_CONTIG) {
+                row = 32 * wave + icl::mfma32_row(lane, r);
+                col = 32 * j + (lane & 31);
+            } else {
+                row = 32 * wave + (lane & 31);
+                col = 32 * j + icl::mfma32_row(lane, r);
+            }
+            if (rb + row < a.nsys && col < ncols) {
+                float* p = a.B + (long)(rb + row) * a.x_si + (long)(c0 + col) * a.x_sj;
+                *p -= u[j][r];
+            }
+        }
+}
+
+// B(i, j) = alpha * X(i, j) through the canonical view of B.
+__global__ void trsm_scatter_kernel(icl_trsm_blocked_args a, float alpha)
+{
+    const long total = (long)a.nsys * a.nlines;
+    const bool rows_fast = a.x_si == 1 || a.x_si == -1;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        long i, j;
+        if (rows_fast) {
+            j = e / a.nsys;
+            i = e - j * a.nsys;
+        } else {
+            i = e / a.nlines;
+            j = e - i * a.nlines;
+        }
+        a.B[i * a.x_si + j * a.x_sj] = alpha * a.X[i + j * a.ldx];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Elementwise 2-D updates.
 // ---------------------------------------------------------------------------
 __global__ void axpby_kernel(int m, int n, float alpha, const float* X, long ldx, float beta, float* Y, long ldy)
@@ -205,6 +371,41 @@ extern "C" hipError_t icl_trsm_diag(const icl_trsm_diag_args* a, hipStream_t str
     if (a->nb > kNB) return hipErrorInvalidValue;
     const unsigned blocks = (unsigned)((a->nlines + kNB - 1) / kNB);
     hipLaunchKernelGGL(trsm_diag_kernel, dim3(blocks), dim3(64), 0, stream, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t icl_trsm_invert_blocks(const icl_trsm_blocked_args* a, hipStream_t stream)
+{
+    if (a->nsys <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((a->nsys + kNB - 1) / kNB);
+    hipLaunchKernelGGL(trsm_invert_kernel, dim3(blocks), dim3(64), 0, stream, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t icl_trsm_step(const icl_trsm_blocked_args* a, int block, hipStream_t stream)
+{
+    const int r0 = block * kNB;
+    if (a->nlines <= 0 || r0 >= a->nsys) return hipSuccess;
+    const int ncb = (a->nlines + kStepCols - 1) / kStepCols;
+    const int below = a->nsys - r0 - kNB;
+    const int groups = below > 0 ? (below + kStepRows - 1) / kStepRows : 1;
+    const bool lines_contig = a->x_sj == 1 || a->x_sj == -1;
+    auto kern = lines_contig ? trsm_step_kernel<true> : trsm_step_kernel<false>;
+    static const hipError_t attr0 = hipFuncSetAttribute(reinterpret_cast<const void*>(trsm_step_kernel<true>),
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStepLds);
+    static const hipError_t attr1 = hipFuncSetAttribute(reinterpret_cast<const void*>(trsm_step_kernel<false>),
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStepLds);
+    if (attr0 != hipSuccess) return attr0;
+    if (attr1 != hipSuccess) return attr1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)(ncb * groups)), dim3(256), kStepLds, stream, *a, block, groups);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t icl_trsm_scatter(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream)
+{
+    if (a->nsys <= 0 || a->nlines <= 0) return hipSuccess;
+    hipLaunchKernelGGL(trsm_scatter_kernel, dim3(elementwise_grid((long)a->nsys * a->nlines)), dim3(256), 0, stream,
+                       *a, alpha);
     return hipGetLastError();
 }
 

@@ -57,7 +57,7 @@ ICLBLAS_API iclblasStatus_t iclblasSynchronize(iclblasHandle_t handle);
 ICLBLAS_API iclblasStatus_t iclblasSetImplementation(iclblasHandle_t handle, const char* name);
 /* Copies the name of the implementation that ran last into buf (NUL-terminated). */
 ICLBLAS_API iclblasStatus_t iclblasGetLastImplementation(iclblasHandle_t handle, char* buf, int len);
-/* Writes the names of all registered GEMM implementations, newline-separated and
+/* Writes the names of all GEMM and STRSM implementations, newline-separated and
  * NUL-terminated, into buf; INVALID_VALUE if len is too small. Needs no handle. */
 ICLBLAS_API iclblasStatus_t iclblasGetImplementations(char* buf, int len);
 

@@ -27,7 +27,9 @@ def vid(v):
 @pytest.fixture
 def h(handle):
     handle.set_math_mode(ib.MATH_FP32)
+    handle.set_implementation(None)
     yield handle
+    handle.set_implementation(None)
 
 
 @pytest.mark.parametrize("case", KATS["strsm"], ids=lambda c: c["name"])
@@ -51,12 +53,14 @@ def _problem(side, uplo, diag, m, n, seed=41, pad=3):
     return A, B
 
 
+@pytest.mark.parametrize("impl", ["strsm_blocked_inv64", "strsm_recursive"])
 @pytest.mark.parametrize("v", VARIANTS, ids=vid)
 @pytest.mark.parametrize("shape", [(5, 3), (64, 64), (65, 17), (200, 130), (300, 77)],
                          ids=lambda s: f"{s[0]}x{s[1]}")
-def test_strsm_variants_vs_fp64(h, v, shape):
+def test_strsm_variants_vs_fp64(h, impl, v, shape):
     side, uplo, trans, diag = v
     m, n = shape
+    h.set_implementation(impl)
     A, B = _problem(side, uplo, diag, m, n)
     alpha = -0.75
     X64 = oracle.dtrsm(side, uplo, trans, diag, m, n, alpha, A, B)
@@ -72,6 +76,8 @@ def test_strsm_variants_vs_fp64(h, v, shape):
     order = m if side == 0 else n
     # within a small multiple of the reference's own float error, or of order*u
     assert err_gpu <= max(8 * err_cpu, 4 * order * 2.0 ** -24), (err_gpu, err_cpu)
+    expect = "strsm_diag64" if order <= 64 else impl
+    assert h.last_implementation() == expect
 
 
 @pytest.mark.parametrize("v", VARIANTS[::3], ids=vid)
