@@ -44,7 +44,9 @@ def parse():
     ap.add_argument("--impl", default="", help="comma list, one implementation name per path (empty = automatic)")
     ap.add_argument("--trans", default="NN", help="NN, NT, TN or TT for the timed op")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check (profiling runs)")
-    ap.add_argument("--no-extra", action="store_true", help="skip the CGEMM / STRSM rows")
+    ap.add_argument("--no-extra", action="store_true", help="skip the CGEMM / STRSM rows (same as --rows sgemm)")
+    ap.add_argument("--rows", default="sgemm,cgemm,strsm",
+                    help="comma list of workloads to run: sgemm (the headline line), cgemm, strsm")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy kernel first (scales HBM byte counters)")
     return ap.parse_args()
@@ -89,15 +91,12 @@ def _time(stream, fn, steps: int, warmup: int) -> float:
     return e0.elapsed_time(e1) / steps
 
 
-def extra_workloads(h, ib, stream, steps: int) -> dict:
-    """The other level-3 rows at their configured shapes: CGEMM 4096^3 NN and CN
-    (alpha = (1, 0.5), beta = (0.25, 0)) on both paths, and STRSM left/lower/N/
-    non-unit 4096 x 1024 plain and with 3 refinement sweeps. Errors against FP64
-    products / solves computed with torch on the GPU."""
+def cgemm_rows(h, ib, stream, steps: int) -> dict:
+    """CGEMM 4096^3 NN and CN (alpha = (1, 0.5), beta = (0.25, 0)) on both
+    paths; errors against an FP64 complex product computed with torch on the GPU."""
     import torch
 
     from clgpu_amd import datagen
-    out = {}
     dev = torch.device("cuda:0")
     n = 4096
     A = torch.from_numpy(np.ascontiguousarray(datagen.complex_matrix(n, n, name="A").T)).to(dev)
@@ -137,11 +136,21 @@ def extra_workloads(h, ib, stream, steps: int) -> dict:
             del err, C, Cw
         cg[opname] = res
         del ref, absab, A64, opAT
-    out["cgemm_4096"] = cg
     h.set_math_mode(ib.MATH_FP32)
     del A, B, C0
     torch.cuda.empty_cache()
+    return cg
 
+
+def strsm_rows(h, ib, stream, steps: int) -> dict:
+    """STRSM left/lower/N/non-unit 4096 x 1024, plain and with 3 refinement
+    sweeps; forward error against an FP64 solve computed with torch on the GPU."""
+    import torch
+
+    from clgpu_amd import datagen
+    dev = torch.device("cuda:0")
+    h.set_math_mode(ib.MATH_FP32)
+    h.set_implementation(None)
     m, nr = 4096, 1024
     T = datagen.triangular(m, lower=True, seed=0, poison=True)
     Bm = datagen.matrix(m, nr, name="B", seed=0)
@@ -162,44 +171,27 @@ def extra_workloads(h, ib, stream, steps: int) -> dict:
         tf = flops_t / (ms * 1e-3) / 1e12
         ts[label] = {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
                      "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA, "forward_rel_err_fro": fe}
-    out["strsm_4096x1024"] = ts
-    return out
+    del tT, tB, L64, X64, X, Xw
+    torch.cuda.empty_cache()
+    return ts
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (this build has no multi-GPU path)",
-              file=sys.stderr)
-        sys.exit(2)
+def sgemm_line(a, h, ib, stream, dev):
+    """The headline SGEMM measurement: both paths, errors, roofline fields.
+    Returns the JSON line and the host inputs (for the CPU baseline)."""
     import torch
 
     from clgpu_amd import datagen
-    from clgpu_amd import iclblas as ib
-
     n = a.size
     m = k = n
     ta, tb = ({"N": 0, "T": 1}[a.trans[0]], {"N": 0, "T": 1}[a.trans[1]])
     print(f"[bench] generating {n}x{n} inputs", file=sys.stderr, flush=True)
     A = datagen.matrix(n, n, name="A", seed=0)
     B = datagen.matrix(n, n, name="B", seed=0)
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     # column-major (ld x cols) <-> torch (cols, ld) row-major
     tA = torch.from_numpy(np.ascontiguousarray(A.T)).to(dev)
     tB = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev)
     tC = torch.empty((n, n), device=dev, dtype=torch.float32)
-    stream = torch.cuda.current_stream()
-    h = ib.Handle(memory_mode=ib.MEMORY_DEVICE, stream=stream.cuda_stream)
-
-    if a.copy_calibration:
-        # a plain copy of known size: 1 GiB read + 1 GiB written
-        src = torch.empty(1 << 28, device=dev, dtype=torch.float32).fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
 
     C64 = absAB = None
     if not a.no_check:
@@ -294,14 +286,61 @@ def main():
                                  "kernel": r["implementation"]}
         if "bf16x3" in results:
             line["split_over_fp32"] = results["bf16x3"]["tflops"] / r["tflops"]
-    if not a.no_extra:
-        print("[bench] CGEMM / STRSM rows", file=sys.stderr, flush=True)
-        del C64, absAB
-        torch.cuda.empty_cache()
-        line["other_rows"] = extra_workloads(h, ib, stream, a.steps)
-    if not a.no_cpu_baseline:
+    del C64, absAB, tA, tB, tC
+    torch.cuda.empty_cache()
+    return line, A, B
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (this build has no multi-GPU path)",
+              file=sys.stderr)
+        sys.exit(2)
+    rows = ["sgemm"] if a.no_extra else [r for r in a.rows.split(",") if r]
+    unknown = set(rows) - {"sgemm", "cgemm", "strsm"}
+    if unknown:
+        print(f"bench.py: unknown rows {sorted(unknown)}", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from clgpu_amd import iclblas as ib
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    h = ib.Handle(memory_mode=ib.MEMORY_DEVICE, stream=stream.cuda_stream)
+
+    if a.copy_calibration:
+        # a plain copy of known size: 1 GiB read + 1 GiB written
+        src = torch.empty(1 << 28, device=dev, dtype=torch.float32).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    A = B = None
+    if "sgemm" in rows:
+        line, A, B = sgemm_line(a, h, ib, stream, dev)
+    else:
+        # a run of the secondary rows only (profiling): no headline value
+        line = {"metric": "SGEMM TFLOP/s @8192: split-BF16 vs FP32-MFMA path, %peak, max err vs FP64",
+                "value": None, "unit": "TFLOP/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32", "data": "synthetic", "config": {"workload": "secondary rows only: " + ",".join(rows)}}
+    other = {}
+    if "cgemm" in rows:
+        print("[bench] CGEMM rows", file=sys.stderr, flush=True)
+        other["cgemm_4096"] = cgemm_rows(h, ib, stream, a.steps)
+    if "strsm" in rows:
+        print("[bench] STRSM rows", file=sys.stderr, flush=True)
+        other["strsm_4096x1024"] = strsm_rows(h, ib, stream, a.steps)
+    if other:
+        line["other_rows"] = other
+    if not a.no_cpu_baseline and A is not None:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(A, B, n, a.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(A, B, a.size, a.cpu_seconds)
     h.close()
     print(json.dumps(line), flush=True)
 

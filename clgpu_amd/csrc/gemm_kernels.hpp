@@ -33,16 +33,17 @@ enum : int {
     kMaskUpperStrict = 4  // keep m < n
 };
 
-__device__ __forceinline__ bool mask_keep(int mask, int m, int n)
-{
-    switch (mask) {
-    case kMaskLower: return m >= n;
-    case kMaskLowerStrict: return m > n;
-    case kMaskUpper: return m <= n;
-    case kMaskUpperStrict: return m < n;
-    default: return true;
+// Triangle mask decoded once per tile: keep(m, n) = !on || (lower ? m - n : n - m) >= strict.
+struct TriMask {
+    bool on, lower;
+    int strict;
+    __device__ __forceinline__ explicit TriMask(int mask)
+        : on(mask != kMaskNone), lower(mask == kMaskLower || mask == kMaskLowerStrict),
+          strict((mask == kMaskLowerStrict || mask == kMaskUpperStrict) ? 1 : 0)
+    {
     }
-}
+    __device__ __forceinline__ bool keep(int m, int n) const { return !on || (lower ? m - n : n - m) >= strict; }
+};
 
 // ---------------------------------------------------------------------------
 // Operand slab loaders.
@@ -399,6 +400,7 @@ template <bool EDGE>
 __device__ __forceinline__ void store_real_tile(const GemmArgs& g, const f32x16& acc, int mrow, int ncol0, int lane)
 {
     // mrow: m index of this lane; ncol0: n index of register 0's 32x32 block origin.
+    const TriMask tri(g.mask);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int n = ncol0 + mfma32_row(lane, r);
@@ -406,8 +408,7 @@ __device__ __forceinline__ void store_real_tile(const GemmArgs& g, const f32x16&
         float* c = g.C + (long)n * g.ldc + mrow;
         float v = g.alpha_re * acc[r];
         if (g.beta_re != 0.0f) v = fmaf(g.beta_re, *c, v);
-        if (g.mask != kMaskNone && !mask_keep(g.mask, mrow, n)) v = 0.0f;
-        *c = v;
+        *c = tri.keep(mrow, n) ? v : 0.0f;
     }
 }
 
