@@ -99,9 +99,11 @@ __global__ void __launch_bounds__(64) trsm_invert_kernel(icl_trsm_blocked_args a
 // ---------------------------------------------------------------------------
 // FP64 residual R = fp32(alpha * B0 - P * Q) on v_mfma_f64_16x16x4_f64.
 // 64 x 64 output tile per workgroup (4 waves, each 32 x 32 = 2 x 2 MFMA blocks),
-// K slabs of 16 staged in LDS as FP64; slabs outside the triangle are skipped.
+// K slabs of 32 staged in LDS as FP64; slabs outside the triangle are skipped.
+// The next slab is read into registers while the current one is multiplied.
 // ---------------------------------------------------------------------------
-constexpr int RT = 64, RK = 16;
+constexpr int RT = 64, RK = 32;
+constexpr int kResPer = RT * RK / 256; // elements of each operand slab per thread
 
 __device__ __forceinline__ float tri_load(const float* X, long ld, int trans, int r, int c, int rows, int cols,
                                           bool tri, int lower, int unit)
@@ -142,21 +144,39 @@ __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    for (int k0 = k_lo; k0 < k_hi; k0 += RK) {
-        // P slab: RT rows x RK k; Q slab: RK k x RT cols. 4 elements per thread each.
+    // P slab: RT rows x RK k; Q slab: RK k x RT cols; the lanes run along the
+    // contiguous dimension of each operand.
+    auto p_coords = [&](int e, int& r, int& k) {
+        if (a.p_trans) { k = e % RK; r = e / RK; } else { r = e % RT; k = e / RT; }
+    };
+    auto q_coords = [&](int e, int& k, int& c) {
+        if (a.q_trans) { c = e % RT; k = e / RT; } else { k = e % RK; c = e / RK; }
+    };
+    float pv[kResPer], qv[kResPer];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int e = tid + 256 * s;
-            int r, k;
-            if (a.p_trans) { k = e % RK; r = e / RK; } else { r = e % RT; k = e / RT; }
-            Ps[k][r] = (double)tri_load(a.P, a.ldp, a.p_trans, m0 + r, k0 + k, a.M, a.K, a.tri_on == 1,
-                                        a.tri_lower, a.tri_unit);
-            int c;
-            if (a.q_trans) { c = e % RT; k = e / RT; } else { k = e % RK; c = e / RK; }
-            Qs[k][c] = (double)tri_load(a.Q, a.ldq, a.q_trans, k0 + k, n0 + c, a.K, a.N, a.tri_on == 2,
-                                        a.tri_lower, a.tri_unit);
+        for (int s = 0; s < kResPer; ++s) {
+            int r, k, c;
+            p_coords(tid + 256 * s, r, k);
+            pv[s] = tri_load(a.P, a.ldp, a.p_trans, m0 + r, k0 + k, a.M, a.K, a.tri_on == 1, a.tri_lower,
+                             a.tri_unit);
+            q_coords(tid + 256 * s, k, c);
+            qv[s] = tri_load(a.Q, a.ldq, a.q_trans, k0 + k, n0 + c, a.K, a.N, a.tri_on == 2, a.tri_lower,
+                             a.tri_unit);
+        }
+    };
+    if (k_lo < k_hi) fetch(k_lo);
+    for (int k0 = k_lo; k0 < k_hi; k0 += RK) {
+#pragma unroll
+        for (int s = 0; s < kResPer; ++s) {
+            int r, k, c;
+            p_coords(tid + 256 * s, r, k);
+            Ps[k][r] = (double)pv[s];
+            q_coords(tid + 256 * s, k, c);
+            Qs[k][c] = (double)qv[s];
         }
         __syncthreads();
+        if (k0 + RK < k_hi) fetch(k0 + RK); // in flight during the products below
 #pragma unroll
         for (int kk = 0; kk < RK; kk += 4) {
             const int kr = kk + (lane >> 4);
@@ -200,30 +220,86 @@ __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
 // ---------------------------------------------------------------------------
 constexpr int kStepCols = 64;  // lines per workgroup
 constexpr int kStepRows = 128; // trailing rows per workgroup
-constexpr int kLdD = kNB + 4, kLdB = kStepCols + 4, kLdL = kStepRows + 4;
+// BX and Lr may be written in transposed order (k fastest across lanes): odd
+// pitches keep those writes free of bank conflicts. D is written as float4.
+constexpr int kLdD = kNB + 4, kLdB = kStepCols + 1, kLdL = kStepRows + 1;
 constexpr size_t kStepLds = sizeof(float) * ((size_t)kNB * kLdD + (size_t)kNB * kLdB + (size_t)kNB * kLdL);
 
-// T[k * LDT + r] = src[r * s_r + k * s_k] for r < R, k < KN (zero beyond rmax / kmax);
-// the thread-to-element map follows whichever of s_r, s_k is the unit stride.
-template <int KN, int R, int LDT>
-__device__ __forceinline__ void load_tile(float* T, const float* src, long s_r, long s_k, int rmax, int kmax, int tid)
-{
-    if (s_r == 1 || s_r == -1) {
-        for (int e = tid; e < KN * R; e += 256) {
-            const int r = e % R, k = e / R;
-            T[k * LDT + r] = (r < rmax && k < kmax) ? src[(long)r * s_r + (long)k * s_k] : 0.0f;
-        }
-    } else {
-        for (int e = tid; e < KN * R; e += 256) {
-            const int k = e % KN, r = e / KN;
-            T[k * LDT + r] = (r < rmax && k < kmax) ? src[(long)r * s_r + (long)k * s_k] : 0.0f;
+// A KN x R tile, element (r, k) = src[r * s_r + k * s_k] (zero beyond rmax /
+// kmax), staged through registers: load() issues every read of the tile before
+// any result is used, so the tile costs one memory latency, not one per
+// element. RFAST: s_r is the unit stride (+1 or -1), so consecutive lanes take
+// consecutive r; otherwise consecutive k. The reads are coalesced either way.
+// Element i of a thread is (r_t, k_t + kStep * i) when RFAST and
+// (r_t + kStep * i, k_t) otherwise, so its address is one per-thread base plus a
+// workgroup-uniform multiple of the slow stride.
+template <int KN, int R, bool RFAST>
+struct TileRegs {
+    static constexpr int kN = KN * R / 256;
+    static constexpr int kStep = 256 / (RFAST ? R : KN);
+    static_assert(256 % R == 0 && 256 % KN == 0, "tile edges must divide 256");
+    float v[kN];
+
+    __device__ __forceinline__ static void coords(int tid, int& r, int& k)
+    {
+        r = RFAST ? tid % R : tid / KN;
+        k = RFAST ? tid / R : tid % KN;
+    }
+    // Elements outside the bounds read *safe (any valid address) and keep 0, so
+    // the reads are straight-line code with no branch between them.
+    __device__ __forceinline__ void load(const float* src, long s_r, long s_k, int rmax, int kmax, int tid,
+                                         const float* safe)
+    {
+        int r, k;
+        coords(tid, r, k);
+        const float* p = src + (long)r * s_r + (long)k * s_k;
+        const long slow_stride = (long)kStep * (RFAST ? s_k : s_r);
+        const bool fast_ok = RFAST ? r < rmax : k < kmax;
+        const int slow0 = RFAST ? k : r, slow_max = RFAST ? kmax : rmax;
+#pragma unroll
+        for (int i = 0; i < kN; ++i) {
+            const bool ok = fast_ok && slow0 + kStep * i < slow_max;
+            const float x = *(ok ? p + i * slow_stride : safe);
+            v[i] = ok ? x : 0.0f;
         }
     }
+    // T[k * LDT + r] = element (r, k)
+    template <int LDT>
+    __device__ __forceinline__ void store(float* T, int tid) const
+    {
+        int r, k;
+        coords(tid, r, k);
+#pragma unroll
+        for (int i = 0; i < kN; ++i) T[(RFAST ? k + kStep * i : k) * LDT + (RFAST ? r : r + kStep * i)] = v[i];
+    }
+};
+
+// Element r of accumulator j of the trailing update sits at (row_t + dr, col_t +
+// dc) relative to (rb, c0), with (row_t, col_t) per thread and (dr, dc)
+// compile-time constants: the lanes run along whichever of rows / lines is
+// contiguous in B (32 x 32 MFMA output layout, rows 8 (r / 4) + 4 (lane / 32) + r % 4).
+template <bool LINES_CONTIG>
+__device__ __forceinline__ void update_base(int wave, int lane, int& row_t, int& col_t)
+{
+    row_t = 32 * wave + (LINES_CONTIG ? 4 * (lane >> 5) : (lane & 31));
+    col_t = LINES_CONTIG ? (lane & 31) : 4 * (lane >> 5);
+}
+template <bool LINES_CONTIG>
+__device__ __forceinline__ constexpr int update_dr(int j, int r)
+{
+    return LINES_CONTIG ? 8 * (r >> 2) + (r & 3) : 0;
+}
+template <bool LINES_CONTIG>
+__device__ __forceinline__ constexpr int update_dc(int j, int r)
+{
+    return 32 * j + (LINES_CONTIG ? 0 : 8 * (r >> 2) + (r & 3));
 }
 
 // LINES_CONTIG: the lines of B are its contiguous dimension (right-side solves),
 // so the trailing update maps lanes to lines for coalesced read-modify-writes.
-template <bool LINES_CONTIG>
+// L_ROWS_CONTIG: the rows of the canonical L are its contiguous dimension
+// (l_si is +1 or -1).
+template <bool LINES_CONTIG, bool L_ROWS_CONTIG>
 __global__ void __launch_bounds__(256) trsm_step_kernel(icl_trsm_blocked_args a, int blk, int groups)
 {
     extern __shared__ __attribute__((aligned(16))) float sm[];
@@ -238,13 +314,41 @@ __global__ void __launch_bounds__(256) trsm_step_kernel(icl_trsm_blocked_args a,
     const int ncols = min(kStepCols, a.nlines - c0);
     const bool update = rb < a.nsys;
 
-    const float* inv = a.invD + (long)blk * kNB * kNB;
-    for (int e = tid; e < kNB * kNB; e += 256) D[(e / kNB) * kLdD + (e % kNB)] = inv[e];
-    load_tile<kNB, kStepCols, kLdB>(BX, a.B + (long)r0 * a.x_si + (long)c0 * a.x_sj, a.x_sj, a.x_si, ncols,
-                                    a.nsys - r0, tid);
-    if (update)
-        load_tile<kNB, kStepRows, kLdL>(Lr, a.L + (long)rb * a.l_si + (long)r0 * a.l_sk, a.l_si, a.l_sk,
-                                        a.nsys - rb, a.nsys - r0, tid);
+    // Every global read of the workgroup is issued up front: inv(L_bb) (dense
+    // 64 x 64, 16-byte aligned), B_b, L_rb and the block of B this workgroup
+    // updates, which no other workgroup of the step touches.
+    // Out-of-range elements (all of L_rb and of the trailing block when there
+    // are no trailing rows) read a valid element of B_b or L_bb instead.
+    const float* b_org = a.B + (long)r0 * a.x_si + (long)c0 * a.x_sj;
+    const float* l_diag = a.L + (long)r0 * (a.l_si + a.l_sk);
+    TileRegs<kNB, kStepCols, LINES_CONTIG> bt;
+    bt.load(b_org, a.x_sj, a.x_si, ncols, a.nsys - r0, tid, b_org);
+    TileRegs<kNB, kStepRows, L_ROWS_CONTIG> lt;
+    lt.load(a.L + (long)rb * a.l_si + (long)r0 * a.l_sk, a.l_si, a.l_sk, a.nsys - rb, a.nsys - r0, tid, l_diag);
+    int row_t, col_t;
+    update_base<LINES_CONTIG>(wave, lane, row_t, col_t);
+    float* const pb = a.B + (long)(rb + row_t) * a.x_si + (long)(c0 + col_t) * a.x_sj;
+    float old[2][16];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int dr = update_dr<LINES_CONTIG>(j, r), dc = update_dc<LINES_CONTIG>(j, r);
+            const bool ok = rb + row_t + dr < a.nsys && col_t + dc < ncols;
+            const float x = *(ok ? pb + dr * a.x_si + dc * a.x_sj : b_org);
+            old[j][r] = ok ? x : 0.0f;
+        }
+    const float4* inv4 = reinterpret_cast<const float4*>(a.invD + (long)blk * kNB * kNB);
+    float4 dv[kNB * kNB / 4 / 256];
+#pragma unroll
+    for (int i = 0; i < kNB * kNB / 4 / 256; ++i) dv[i] = inv4[tid + 256 * i];
+    bt.template store<kLdB>(BX, tid);
+    if (update) lt.template store<kLdL>(Lr, tid);
+#pragma unroll
+    for (int i = 0; i < kNB * kNB / 4 / 256; ++i) {
+        const int e = 4 * (tid + 256 * i);
+        *reinterpret_cast<float4*>(D + (e / kNB) * kLdD + (e % kNB)) = dv[i];
+    }
     __syncthreads();
 
     // X_b quadrant of this wave: rows wr.., lines wc.. (lane <-> row)
@@ -284,18 +388,8 @@ __global__ void __launch_bounds__(256) trsm_step_kernel(icl_trsm_blocked_args a,
     for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            int row, col;
-            if (LINES_CONTIG) {
-                row = 32 * wave + icl::mfma32_row(lane, r);
-                col = 32 * j + (lane & 31);
-            } else {
-                row = 32 * wave + (lane & 31);
-                col = 32 * j + icl::mfma32_row(lane, r);
-            }
-            if (rb + row < a.nsys && col < ncols) {
-                float* p = a.B + (long)(rb + row) * a.x_si + (long)(c0 + col) * a.x_sj;
-                *p -= u[j][r];
-            }
+            const int dr = update_dr<LINES_CONTIG>(j, r), dc = update_dc<LINES_CONTIG>(j, r);
+            if (rb + row_t + dr < a.nsys && col_t + dc < ncols) pb[dr * a.x_si + dc * a.x_sj] = old[j][r] - u[j][r];
         }
 }
 
@@ -389,14 +483,21 @@ extern "C" hipError_t icl_trsm_step(const icl_trsm_blocked_args* a, int block, h
     const int ncb = (a->nlines + kStepCols - 1) / kStepCols;
     const int below = a->nsys - r0 - kNB;
     const int groups = below > 0 ? (below + kStepRows - 1) / kStepRows : 1;
+    using Kern = void (*)(icl_trsm_blocked_args, int, int);
+    static const Kern kernels[4] = {trsm_step_kernel<false, false>, trsm_step_kernel<false, true>,
+                                    trsm_step_kernel<true, false>, trsm_step_kernel<true, true>};
+    static const hipError_t attr = [] {
+        for (Kern k : kernels) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStepLds);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }();
+    if (attr != hipSuccess) return attr;
     const bool lines_contig = a->x_sj == 1 || a->x_sj == -1;
-    auto kern = lines_contig ? trsm_step_kernel<true> : trsm_step_kernel<false>;
-    static const hipError_t attr0 = hipFuncSetAttribute(reinterpret_cast<const void*>(trsm_step_kernel<true>),
-                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStepLds);
-    static const hipError_t attr1 = hipFuncSetAttribute(reinterpret_cast<const void*>(trsm_step_kernel<false>),
-                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStepLds);
-    if (attr0 != hipSuccess) return attr0;
-    if (attr1 != hipSuccess) return attr1;
+    const bool l_rows_contig = a->l_si == 1 || a->l_si == -1;
+    const Kern kern = kernels[2 * lines_contig + l_rows_contig];
     hipLaunchKernelGGL(kern, dim3((unsigned)(ncb * groups)), dim3(256), kStepLds, stream, *a, block, groups);
     return hipGetLastError();
 }
