@@ -174,6 +174,10 @@ void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int 
 namespace {
 
 constexpr int kLeaf = 64;
+// Whether the single-launch dataflow solve, rather than the launch-per-step
+// blocked solve, runs when no implementation is forced (both are selectable
+// by name).
+constexpr bool kTrsmDataflowDefault = false;
 
 struct Trsm {
     iclblasContext* h;
@@ -343,16 +347,53 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
         }
         const int nblk = (order + kLeaf - 1) / kLeaf;
         const size_t inv_elems = (size_t)nblk * kLeaf * kLeaf;
-        float* ws = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
-            (inv_elems + (size_t)order * a.nlines) * sizeof(float), h->stream));
-        a.invD = ws;
-        a.X = ws + inv_elems;
-        a.ldx = order;
-        hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
-        for (int b = 0; b < nblk; ++b) hip_check(icl_trsm_step(&a, b, h->stream), "trsm_step");
-        hip_check(icl_trsm_scatter(&a, alpha, h->stream), "trsm_scatter");
-        if (record) h->last_impl = "strsm_blocked_inv64";
+        const bool dataflow = h->forced_impl == "strsm_dataflow64" ||
+                              (kTrsmDataflowDefault && h->forced_impl != "strsm_blocked_inv64");
+        if (dataflow) {
+            // invD, the tile-major X (whole 64 x 64 tiles), then the flags / ticket / abort words
+            const int ncb = (a.nlines + kLeaf - 1) / kLeaf;
+            const size_t x_elems = (size_t)nblk * ncb * kLeaf * kLeaf;
+            const size_t sync_ints = (size_t)icl_trsm_dataflow_sync_ints(order, a.nlines);
+            char* ws = static_cast<char*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
+                (inv_elems + x_elems) * sizeof(float) + sync_ints * sizeof(int), h->stream));
+            a.invD = reinterpret_cast<float*>(ws);
+            a.X = a.invD + inv_elems;
+            a.ldx = 0;
+            a.sync = reinterpret_cast<int*>(a.X + x_elems);
+            if (!h->trsm_error) {
+                void* w = nullptr;
+                hip_check(hipMalloc(&w, 256), "hipMalloc");
+                h->trsm_error = static_cast<int*>(w);
+                hip_check(icl_axpby_2d(64, 1, 0.0f, nullptr, 0, 0.0f, static_cast<float*>(w), 64, h->stream), "axpby");
+            }
+            a.error_word = h->trsm_error;
+            hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
+            hip_check(icl_trsm_dataflow(&a, alpha, h->stream), "trsm_dataflow");
+            if (record) h->last_impl = "strsm_dataflow64";
+        } else {
+            float* ws = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
+                (inv_elems + (size_t)order * a.nlines) * sizeof(float), h->stream));
+            a.invD = ws;
+            a.X = ws + inv_elems;
+            a.ldx = order;
+            hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
+            for (int b = 0; b < nblk; ++b) hip_check(icl_trsm_step(&a, b, h->stream), "trsm_step");
+            hip_check(icl_trsm_scatter(&a, alpha, h->stream), "trsm_scatter");
+            if (record) h->last_impl = "strsm_blocked_inv64";
+        }
     }
+}
+
+void trsm_check_error(iclblasContext* h)
+{
+    if (!h->trsm_error) return;
+    int e = 0;
+    hip_check(hipMemcpy(&e, h->trsm_error, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (e == 0) return;
+    hip_check(icl_axpby_2d(64, 1, 0.0f, nullptr, 0, 0.0f, reinterpret_cast<float*>(h->trsm_error), 64, h->stream),
+              "axpby");
+    hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    throw status_error(ICLBLAS_STATUS_EXECUTION_FAILED, "STRSM dataflow solve exceeded its wait bound");
 }
 
 void strsm_refined_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,

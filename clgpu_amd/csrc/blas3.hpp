@@ -46,6 +46,11 @@ void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int 
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
                const float* A, long lda, float* B, long ldb, bool record = true);
 
+// Host memory mode only (after the stream is synchronised): throws
+// EXECUTION_FAILED, and clears the word, when a dataflow STRSM of this handle
+// aborted a wait.
+void trsm_check_error(iclblasContext* h);
+
 // STRSM + FP64-residual iterative refinement. ws0/ws1: m*n floats each.
 void strsm_refined_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
                        const float* A, long lda, float* B, long ldb, int sweeps, float* ws0, float* ws1);

@@ -31,7 +31,7 @@ enum Slot { kSlotA = 0, kSlotB = 1, kSlotC = 2, kSlotW0 = 3, kSlotW1 = 4, kSlotW
 // STRSM implementations (selected by size; forceable by name): the blocked solve
 // with inverted 64 x 64 diagonal blocks (default above 64 unknowns) and the
 // recursive split into diagonal-block solves plus GEMM updates.
-const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_recursive"};
+const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_dataflow64", "strsm_recursive"};
 
 template <typename F>
 iclblasStatus_t guarded(F&& f)
@@ -169,6 +169,7 @@ ICLBLAS_API iclblasStatus_t iclblasDestroy(iclblasHandle_t handle)
     (void)hipSetDevice(handle->device);
     (void)hipStreamSynchronize(handle->stream);
     for (auto& s : handle->scratch) s.release();
+    if (handle->trsm_error) (void)hipFree(handle->trsm_error);
     if (handle->own_stream) (void)hipStreamDestroy(handle->stream);
     handle->tag = 0;
     delete handle;
@@ -265,6 +266,7 @@ static iclblasStatus_t strsm_common(iclblasHandle_t handle, iclblasSideMode_t si
         }
         if (host) stage_out<float>(h, B, dB, m, n, ldb);
         finish(h);
+        if (host) icl::trsm_check_error(h);
     });
 }
 
@@ -467,6 +469,7 @@ ICLBLAS_API iclblasStatus_t iclblasStrsmBackward(iclblasHandle_t handle, iclblas
             if (dB) stage_out<float>(h, dB, hdB, m, n, lddb);
         }
         finish(h);
+        if (h->memory_mode == ICLBLAS_MEMORY_HOST) icl::trsm_check_error(h);
     });
 }
 

@@ -86,6 +86,7 @@ struct iclblasContext {
     std::string last_impl;
     std::mutex mu; // a handle serializes its own calls
     icl::DeviceScratch scratch[kScratchSlots];
+    int* trsm_error = nullptr; // device word set by an aborted dataflow STRSM (allocated on first use)
 
     static bool valid(const iclblasContext* h) { return h != nullptr && h->tag == kTag; }
 };

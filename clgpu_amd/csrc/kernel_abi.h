@@ -95,10 +95,26 @@ typedef struct icl_trsm_blocked_args {
     long ldx;
     float* invD;
     int unit;
+    int* sync;       /* dataflow solve only: icl_trsm_dataflow_sync_ints(...) ints, zeroed by invert_blocks */
+    int* error_word; /* dataflow solve only: set to 1 (never cleared by the kernels) when a wait is aborted */
 } icl_trsm_blocked_args;
 hipError_t icl_trsm_invert_blocks(const icl_trsm_blocked_args* a, hipStream_t stream);
 hipError_t icl_trsm_step(const icl_trsm_blocked_args* a, int block, hipStream_t stream);
 hipError_t icl_trsm_scatter(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream);
+
+/* The whole blocked solve in one launch. X is tile-major here: the 64 x 64
+ * tile (row block j, line block c) holds X(64 j + k, 64 c + l) at
+ * X[(j * ncb + c) * 4096 + k * 64 + l], ncb = ceil(nlines / 64); ldx is
+ * unused. One workgroup per tile takes tiles in row-block order from a ticket
+ * counter, accumulates L_bj X_jc over j < b as the producers' flags are set,
+ * then X_bc = inv(L_bb) (B_bc - sum), stores it, sets its flag and writes
+ * alpha * X_bc through the view of B. invert_blocks (same stream, just before)
+ * zeroes a->sync (per-tile flags, the ticket counter, then the abort word). A
+ * wait that exceeds its spin bound sets the abort word (the last int), after
+ * which every wait returns at once, so the launch always drains; the result is
+ * then invalid, and *error_word is set for the host to report. */
+int icl_trsm_dataflow_sync_ints(int nsys, int nlines);
+hipError_t icl_trsm_dataflow(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream);
 
 /* R = fp32(alpha * B0 - P * Q) with the products and the sum in FP64.
  * P is M x K, Q is K x N (fp32, column-major, optionally transposed); the

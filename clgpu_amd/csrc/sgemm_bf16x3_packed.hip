@@ -1,8 +1,9 @@
 // Split-BF16 (x3) SGEMM on pre-split, pre-packed operands.
 //
-// A prepass (split_pack_kernel) reads op(A) (m x k) and op(B)^T (n x k) once,
-// splits every FP32 x into hi = bf16(x), lo = bf16(x - hi), and writes them in
-// a slab-major layout that is exactly the LDS image the GEMM wants:
+// A prepass (split_pack_kernel, one launch for both operands) reads op(A)
+// (m x k) and op(B)^T (n x k) once, splits every FP32 x into hi = bf16(x),
+// lo = bf16(x - hi), and writes them in a slab-major layout that is exactly the
+// LDS image the GEMM wants:
 //
 //   P[ks][r][h][kk]   ks = k / 32, r = row (m for A, n for B), h = 0 hi / 1 lo,
 //                     kk = k % 32;  offset ((ks * Rpad + r) * 2 + h) * 32 + kk
@@ -20,47 +21,67 @@ namespace icl {
 constexpr int kPackK = 32; // k per slab
 
 // ---------------------------------------------------------------------------
-// Prepass: 64 rows x 32 k per 256-thread block, transposed through LDS when
-// the rows are the contiguous dimension of the source.
-// element (r, k) of the source is X[(k) + r * ld] when KCONTIG, else X[r + k * ld].
+// Prepass: both operands in one launch. Workgroup (x, y) packs 64 rows of
+// op(A) (x < ablocks) or of op(B)^T (the rest) over k = 64 y .. 64 y + 63, i.e.
+// two slabs, staged through LDS. Element (r, k) of a source is X[k + r * ld]
+// when kcontig, else X[r + k * ld]; the branch on it is uniform per workgroup.
+// All 16 reads of a thread are issued before any is stored.
 // ---------------------------------------------------------------------------
-template <bool KCONTIG>
-__global__ void __launch_bounds__(256)
-split_pack_kernel(const float* __restrict__ X, long ld, int R, int K, int Rpad, __bf16* __restrict__ P)
+struct PackSrc {
+    const float* X;
+    long ld;
+    int R, K, Rpad, kcontig;
+    __bf16* P;
+};
+
+__global__ void __launch_bounds__(256) split_pack_kernel(PackSrc a, PackSrc b, int ablocks, int slabs)
 {
-    __shared__ float T[64][kPackK + 1];
-    const int r0 = blockIdx.x * 64;
-    const int ks = blockIdx.y;
-    const int k0 = ks * kPackK;
+    __shared__ float T[64][64 + 1];
+    const bool is_a = (int)blockIdx.x < ablocks;
+    const PackSrc o = is_a ? a : b;
+    const int r0 = (is_a ? (int)blockIdx.x : (int)blockIdx.x - ablocks) * 64;
+    const int k0 = blockIdx.y * 64;
     const int t = threadIdx.x;
-    if constexpr (!KCONTIG) {
-        const int r = t & 63, kb = t >> 6;
+    float v[16];
+    if (o.kcontig) {
+        const int k = t & 63, rb = t >> 6;
+        const bool kok = k0 + k < o.K;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = kb + 4 * i;
-            T[r][k] = (r0 + r < R && k0 + k < K) ? X[(long)(k0 + k) * ld + r0 + r] : 0.0f;
+        for (int i = 0; i < 16; ++i) {
+            const int r = rb + 4 * i;
+            v[i] = (kok && r0 + r < o.R) ? o.X[(long)(r0 + r) * o.ld + k0 + k] : 0.0f;
         }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) T[rb + 4 * i][k] = v[i];
     } else {
-        const int k = t & 31, rb = t >> 5;
+        const int r = t & 63, kb = t >> 6;
+        const bool rok = r0 + r < o.R;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int r = rb + 8 * i;
-            T[r][k] = (r0 + r < R && k0 + k < K) ? X[(long)(r0 + r) * ld + k0 + k] : 0.0f;
+        for (int i = 0; i < 16; ++i) {
+            const int k = kb + 4 * i;
+            v[i] = (rok && k0 + k < o.K) ? o.X[(long)(k0 + k) * o.ld + r0 + r] : 0.0f;
         }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) T[r][kb + 4 * i] = v[i];
     }
     __syncthreads();
-    const int r = t >> 2, kq = (t & 3) * 8;
-    bf16x8 hi, lo;
+    // thread -> row r, 16 consecutive k from kq: slab 2y + kq / 32, in-slab offset kq % 32
+    const int r = t >> 2, kq = (t & 3) * 16;
+    const int ks = 2 * (int)blockIdx.y + kq / kPackK;
+    if (ks >= slabs) return;
+    bf16x8 hi[2], lo[2];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 16; ++i) {
         __bf16 h, l;
         split_bf16(T[r][kq + i], h, l);
-        hi[i] = h;
-        lo[i] = l;
+        hi[i / 8][i % 8] = h;
+        lo[i / 8][i % 8] = l;
     }
-    __bf16* dst = P + ((size_t)ks * Rpad + r0 + r) * (2 * kPackK) + kq;
-    *reinterpret_cast<bf16x8*>(dst) = hi;
-    *reinterpret_cast<bf16x8*>(dst + kPackK) = lo;
+    __bf16* dst = o.P + ((size_t)ks * o.Rpad + r0 + r) * (2 * kPackK) + kq % kPackK;
+    *reinterpret_cast<bf16x8*>(dst) = hi[0];
+    *reinterpret_cast<bf16x8*>(dst + 8) = hi[1];
+    *reinterpret_cast<bf16x8*>(dst + kPackK) = lo[0];
+    *reinterpret_cast<bf16x8*>(dst + kPackK + 8) = lo[1];
 }
 
 struct PackedArgs {
@@ -204,11 +225,11 @@ size_t packed_workspace(const icl_gemm_args* g)
     return ((size_t)round_up(g->M, BM) + (size_t)round_up(g->N, BN)) * kpad * 2 * sizeof(__bf16);
 }
 
-hipError_t pack(const float* X, long ld, int R, int K, int Rpad, int kcontig, __bf16* P, hipStream_t s)
+hipError_t pack(const PackSrc& a, const PackSrc& b, int slabs, hipStream_t s)
 {
-    const dim3 grid((unsigned)(Rpad / 64), (unsigned)(round_up(K, kPackK) / kPackK));
-    if (kcontig) hipLaunchKernelGGL(split_pack_kernel<true>, grid, dim3(256), 0, s, X, ld, R, K, Rpad, P);
-    else hipLaunchKernelGGL(split_pack_kernel<false>, grid, dim3(256), 0, s, X, ld, R, K, Rpad, P);
+    const int ablocks = a.Rpad / 64;
+    const dim3 grid((unsigned)(ablocks + b.Rpad / 64), (unsigned)((slabs + 1) / 2));
+    hipLaunchKernelGGL(split_pack_kernel, grid, dim3(256), 0, s, a, b, ablocks, slabs);
     return hipGetLastError();
 }
 
@@ -226,10 +247,10 @@ hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, voi
     p.Ap = Ap;
     p.Bp = Bp;
     // op(A)(m, k): N -> A[m + k*lda] (rows contiguous), T/C -> A[k + m*lda] (k contiguous)
-    hipError_t e = pack(g->A, g->lda, g->M, g->K, p.Mpad, opa != 0, Ap, s);
-    if (e != hipSuccess) return e;
+    const PackSrc pa{g->A, g->lda, g->M, g->K, p.Mpad, opa != 0, Ap};
     // op(B)(k, n) as rows n: N -> B[k + n*ldb] (k contiguous), T/C -> B[n + k*ldb]
-    e = pack(g->B, g->ldb, g->N, g->K, p.Npad, opb == 0, Bp, s);
+    const PackSrc pb{g->B, g->ldb, g->N, g->K, p.Npad, opb == 0, Bp};
+    const hipError_t e = pack(pa, pb, p.slabs, s);
     if (e != hipSuccess) return e;
     const bool even = g->M % BM == 0 && g->N % BN == 0;
     (void)edge;
@@ -261,5 +282,6 @@ static const icl_gemm_variant kTable[] = {
     ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8", 256, 256, 4, 2, 405.8f, 1),
     ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8n", 256, 256, 2, 4, 400.8f, 1),
     ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, 344.6f, 2),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w4", 256, 256, 2, 2, 400.0f, 1),
 };
 ICL_REGISTER_VARIANTS(kTable)

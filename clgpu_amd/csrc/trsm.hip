@@ -94,37 +94,60 @@ __global__ void __launch_bounds__(64) trsm_invert_kernel(icl_trsm_blocked_args a
     float* col = a.invD + (long)b * kNB * kNB + (long)lane * kNB;
 #pragma unroll
     for (int i = 0; i < kNB; ++i) col[i] = (lane < nb && i < nb) ? x[i] : 0.0f;
+    if (a.sync && b == 0) { // the dataflow solve that follows on this stream starts from zeroed flags
+        const int n = ((a.nsys + kNB - 1) / kNB) * ((a.nlines + kNB - 1) / kNB) + 2;
+        for (int i = lane; i < n; i += kNB) a.sync[i] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // FP64 residual R = fp32(alpha * B0 - P * Q) on v_mfma_f64_16x16x4_f64.
 // 64 x 64 output tile per workgroup (4 waves, each 32 x 32 = 2 x 2 MFMA blocks),
-// K slabs of 32 staged in LDS as FP64; slabs outside the triangle are skipped.
-// The next slab is read into registers while the current one is multiplied.
+// K slabs of 32 staged in LDS as FP64; slabs outside the triangle are skipped
+// and only slabs that cross the diagonal apply the triangle rule. The next slab
+// is read into registers (straight-line, strength-reduced addressing) while the
+// current one is multiplied. Tiles are dispatched longest first (their K range
+// depends on their position relative to the triangle), so the short ones fill
+// the tail.
 // ---------------------------------------------------------------------------
 constexpr int RT = 64, RK = 32;
 constexpr int kResPer = RT * RK / 256; // elements of each operand slab per thread
 
-__device__ __forceinline__ float tri_load(const float* X, long ld, int trans, int r, int c, int rows, int cols,
-                                          bool tri, int lower, int unit)
-{
-    // element (r, c) of op(X) (rows x cols), with the triangle rule applied
-    if (r >= rows || c >= cols) return 0.0f;
-    if (tri) {
-        if (r == c && unit) return 1.0f;
-        if (lower ? (c > r) : (c < r)) return 0.0f;
+// One thread's share of a slab of an operand whose element (i, k) (i < RT along
+// the tile, k < RK along K) sits at X[i * s_i + k * s_k]. KFAST: k is the unit
+// stride (s_k == 1), so consecutive lanes take consecutive k. Element s of the
+// thread is (i0 + s * di, k0 + s * dk) with one of di, dk zero.
+template <bool KFAST>
+struct ResSlab {
+    static constexpr int kDi = KFAST ? 256 / RK : 0, kDk = KFAST ? 0 : 256 / RT;
+    int i0, k0;
+    __device__ __forceinline__ explicit ResSlab(int tid)
+    {
+        i0 = KFAST ? tid / RK : tid % RT;
+        k0 = KFAST ? tid % RK : tid / RT;
     }
-    return trans ? X[(long)r * ld + c] : X[(long)c * ld + r];
-}
+};
 
+template <bool PT, bool QT>
 __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
 {
-    __shared__ double Ps[RK][RT + 1];
-    __shared__ double Qs[RK][RT + 1];
+    __shared__ double Ps[RK][RT + 1]; // Ps[k][r] = P(m0 + r, kb + k)
+    __shared__ double Qs[RK][RT + 1]; // Qs[k][c] = Q(kb + k, n0 + c)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tiles_m = (a.M + RT - 1) / RT;
-    const int m0 = (blockIdx.x % tiles_m) * RT;
-    const int n0 = (blockIdx.x / tiles_m) * RT;
+    const int tiles_m = (a.M + RT - 1) / RT, tiles_n = (a.N + RT - 1) / RT;
+    const int bid = blockIdx.x;
+    int tm, tn;
+    if (a.tri_on == 1) { // K range grows with m (lower) or shrinks (upper)
+        tn = bid % tiles_n;
+        tm = a.tri_lower ? tiles_m - 1 - bid / tiles_n : bid / tiles_n;
+    } else if (a.tri_on == 2) { // K range shrinks with n (lower) or grows (upper)
+        tm = bid % tiles_m;
+        tn = a.tri_lower ? bid / tiles_m : tiles_n - 1 - bid / tiles_m;
+    } else {
+        tm = bid % tiles_m;
+        tn = bid / tiles_m;
+    }
+    const int m0 = tm * RT, n0 = tn * RT;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
 
     // K range that can be non-zero for this tile.
@@ -144,39 +167,54 @@ __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    // P slab: RT rows x RK k; Q slab: RK k x RT cols; the lanes run along the
-    // contiguous dimension of each operand.
-    auto p_coords = [&](int e, int& r, int& k) {
-        if (a.p_trans) { k = e % RK; r = e / RK; } else { r = e % RT; k = e / RT; }
-    };
-    auto q_coords = [&](int e, int& k, int& c) {
-        if (a.q_trans) { c = e % RT; k = e / RT; } else { k = e % RK; c = e / RK; }
-    };
+    // P(i, k) = P[i * ldp + k] when PT, P[i + k * ldp] otherwise; likewise Q(k, c)
+    // = Q[k * ldq + c] when QT, Q[c * ldq + k] otherwise (c plays the role of i).
+    const ResSlab<PT> ps(tid);
+    const ResSlab<!QT> qs(tid);
+    const long p_si = PT ? a.ldp : 1, p_sk = PT ? 1 : a.ldp;
+    const long q_si = QT ? 1 : a.ldq, q_sk = QT ? a.ldq : 1;
+    const int pi = m0 + ps.i0, qi = n0 + qs.i0;
     float pv[kResPer], qv[kResPer];
-    auto fetch = [&](int k0) {
+    auto fetch = [&](int kb) {
+        // triangle rule only where the slab crosses the diagonal of the triangular operand
+        const bool p_diag = a.tri_on == 1 && kb < m0 + RT && m0 < kb + RK;
+        const bool q_diag = a.tri_on == 2 && kb < n0 + RT && n0 < kb + RK;
+        const float* pp = a.P + (long)pi * p_si + (long)(kb + ps.k0) * p_sk;
+        const float* qp = a.Q + (long)qi * q_si + (long)(kb + qs.k0) * q_sk;
 #pragma unroll
         for (int s = 0; s < kResPer; ++s) {
-            int r, k, c;
-            p_coords(tid + 256 * s, r, k);
-            pv[s] = tri_load(a.P, a.ldp, a.p_trans, m0 + r, k0 + k, a.M, a.K, a.tri_on == 1, a.tri_lower,
-                             a.tri_unit);
-            q_coords(tid + 256 * s, k, c);
-            qv[s] = tri_load(a.Q, a.ldq, a.q_trans, k0 + k, n0 + c, a.K, a.N, a.tri_on == 2, a.tri_lower,
-                             a.tri_unit);
+            const int i = pi + s * ps.kDi, k = kb + ps.k0 + s * ps.kDk;
+            const bool ok = i < a.M && k < a.K;
+            const float x = *(ok ? pp + (long)s * (ps.kDi * p_si + ps.kDk * p_sk) : a.P);
+            float v = ok ? x : 0.0f;
+            if (p_diag) {
+                v = (a.tri_lower ? k > i : k < i) ? 0.0f : v;
+                v = (a.tri_unit && k == i) ? 1.0f : v;
+            }
+            pv[s] = v;
+        }
+#pragma unroll
+        for (int s = 0; s < kResPer; ++s) {
+            const int c = qi + s * qs.kDi, k = kb + qs.k0 + s * qs.kDk;
+            const bool ok = c < a.N && k < a.K;
+            const float x = *(ok ? qp + (long)s * (qs.kDi * q_si + qs.kDk * q_sk) : a.Q);
+            float v = ok ? x : 0.0f;
+            if (q_diag) { // Q(k, c): row k, column c of the triangle
+                v = (a.tri_lower ? c > k : c < k) ? 0.0f : v;
+                v = (a.tri_unit && k == c) ? 1.0f : v;
+            }
+            qv[s] = v;
         }
     };
     if (k_lo < k_hi) fetch(k_lo);
-    for (int k0 = k_lo; k0 < k_hi; k0 += RK) {
+    for (int kb = k_lo; kb < k_hi; kb += RK) {
 #pragma unroll
         for (int s = 0; s < kResPer; ++s) {
-            int r, k, c;
-            p_coords(tid + 256 * s, r, k);
-            Ps[k][r] = (double)pv[s];
-            q_coords(tid + 256 * s, k, c);
-            Qs[k][c] = (double)qv[s];
+            Ps[ps.k0 + s * ps.kDk][ps.i0 + s * ps.kDi] = (double)pv[s];
+            Qs[qs.k0 + s * qs.kDk][qs.i0 + s * qs.kDi] = (double)qv[s];
         }
         __syncthreads();
-        if (k0 + RK < k_hi) fetch(k0 + RK); // in flight during the products below
+        if (kb + RK < k_hi) fetch(kb + RK); // in flight during the products below
 #pragma unroll
         for (int kk = 0; kk < RK; kk += 4) {
             const int kr = kk + (lane >> 4);
@@ -393,6 +431,170 @@ __global__ void __launch_bounds__(256) trsm_step_kernel(icl_trsm_blocked_args a,
         }
 }
 
+// ---------------------------------------------------------------------------
+// Dataflow blocked solve: the whole solve in one launch, one workgroup per
+// 64 x 64 tile (row block b, line block c) of X:
+//   X_bc = inv(L_bb) (B_bc - sum_{j<b} L_bj X_jc)
+// Tiles are taken in row-block order from a ticket counter, so a workgroup only
+// waits for tiles whose tickets are smaller; those were taken by workgroups
+// that are already running and themselves wait only on smaller tickets, so
+// every wait ends (by induction from ticket 0) however many workgroups are
+// resident. Each wait is also bounded: past kDfSpin polls it sets the abort
+// word, and every later wait returns at once, so the launch always drains.
+// Producers publish a tile with release fences at agent scope before setting
+// its flag; consumers acquire at agent scope after seeing it (the tiles may be
+// produced on another XCD, whose L2 is not coherent with this one).
+// X is tile-major; a tile stores its 64 x 64 values with the dimension that is
+// contiguous in B fastest (lines when LINES_CONTIG, rows otherwise).
+// ---------------------------------------------------------------------------
+constexpr int kDfLd = kNB + 1;                                      // pitch of the [k][r] LDS tiles
+constexpr int kDfSpin = 1 << 16;                                    // polls per wait before aborting
+constexpr size_t kDfLds = sizeof(float) * ((size_t)2 * kNB * kDfLd + (size_t)kNB * kLdD);
+
+__device__ __forceinline__ int df_poll(const int* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every lane of a wave calls this with the same flag. Returns false when the
+// solve was aborted (then nothing may be assumed about the tile).
+__device__ __forceinline__ bool df_wait(const int* flag, int* abort_word, int* error_word)
+{
+    for (int it = 0;; ++it) {
+        if (df_poll(flag) != 0) break;
+        if (df_poll(abort_word) != 0) return false;
+        if (it >= kDfSpin) {
+            __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(error_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+template <bool LINES_CONTIG>
+__device__ __forceinline__ f32x16 df_mfma(float row_frag, float line_frag, f32x16 acc)
+{
+    // lanes of the result run along lines when LINES_CONTIG, along rows otherwise
+    return LINES_CONTIG ? icl::mfma_32x32x2_f32(row_frag, line_frag, acc)
+                        : icl::mfma_32x32x2_f32(line_frag, row_frag, acc);
+}
+
+template <bool LINES_CONTIG, bool L_ROWS_CONTIG>
+__global__ void __launch_bounds__(256) trsm_dataflow_kernel(icl_trsm_blocked_args a, float alpha)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    float* Ds = sm;                  // Ds[k][r] = inv(L_bb)(r, k), pitch kLdD
+    float* Ls = Ds + kNB * kLdD;     // Ls[k][r] = L(64 b + r, 64 j + k)
+    float* Xs = Ls + kNB * kDfLd;    // Xs[k][l] = X_jc(k, l); at the end R(k, l)
+    __shared__ int s_ticket;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nblk = (a.nsys + kNB - 1) / kNB, ncb = (a.nlines + kNB - 1) / kNB;
+    int* const flags = a.sync;
+    int* const ticket = a.sync + nblk * ncb;
+    int* const abort_word = ticket + 1;
+    if (tid == 0) s_ticket = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int t = s_ticket;
+    if (t >= nblk * ncb) return; // not reached: the grid has exactly one workgroup per tile
+    const int b = t / ncb, c = t % ncb;
+    const int r0 = b * kNB, c0 = c * kNB;
+    const int ncols = min(kNB, a.nlines - c0);
+    // tile-major X: element (k, l) of a tile at k * s_k + l * s_l
+    constexpr long s_l = LINES_CONTIG ? 1 : kNB, s_k = LINES_CONTIG ? kNB : 1;
+    auto xtile = [&](int j) { return a.X + ((long)j * ncb + c) * (kNB * kNB); };
+
+    // Static operands first: inv(L_bb) into LDS and this tile's B values in the
+    // accumulator layout (no other tile reads or writes B_bc).
+    const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1); // quadrant of this wave: rows wr.., lines wc..
+    const int row_t = wr + (LINES_CONTIG ? 4 * (lane >> 5) : (lane & 31));
+    const int col_t = wc + (LINES_CONTIG ? (lane & 31) : 4 * (lane >> 5));
+    const float* b_org = a.B + (long)r0 * a.x_si + (long)c0 * a.x_sj;
+    float* const pb = a.B + (long)(r0 + row_t) * a.x_si + (long)(c0 + col_t) * a.x_sj;
+    float bv[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int d = 8 * (r >> 2) + (r & 3);
+        const int dr = LINES_CONTIG ? d : 0, dc = LINES_CONTIG ? 0 : d;
+        const bool ok = r0 + row_t + dr < a.nsys && col_t + dc < ncols;
+        const float x = *(ok ? pb + dr * a.x_si + dc * a.x_sj : b_org);
+        bv[r] = ok ? x : 0.0f;
+    }
+    {
+        const float4* inv4 = reinterpret_cast<const float4*>(a.invD + (long)b * kNB * kNB);
+#pragma unroll
+        for (int i = 0; i < kNB * kNB / 4 / 256; ++i) {
+            const int e = 4 * (tid + 256 * i);
+            *reinterpret_cast<float4*>(Ds + (e / kNB) * kLdD + (e % kNB)) = inv4[tid + 256 * i];
+        }
+    }
+
+    // sum_{j<b} L_bj X_jc, the next pair of tiles read into registers while the
+    // current one is multiplied
+    const float* l_row = a.L + (long)r0 * a.l_si;
+    const float* l_diag = l_row + (long)r0 * a.l_sk;
+    TileRegs<kNB, kNB, L_ROWS_CONTIG> lt;
+    TileRegs<kNB, kNB, LINES_CONTIG> xt;
+    f32x16 acc = f32x16{};
+    if (b > 0) {
+        lt.load(l_row, a.l_si, a.l_sk, a.nsys - r0, kNB, tid, l_diag);
+        df_wait(&flags[c], abort_word, a.error_word);
+        xt.load(xtile(0), s_l, s_k, kNB, kNB, tid, a.X);
+    }
+    for (int j = 0; j < b; ++j) {
+        __syncthreads(); // every wave is done with the tiles of step j - 1
+        lt.template store<kDfLd>(Ls, tid);
+        xt.template store<kDfLd>(Xs, tid);
+        __syncthreads();
+        if (j + 1 < b) {
+            lt.load(l_row + (long)(j + 1) * kNB * a.l_sk, a.l_si, a.l_sk, a.nsys - r0, kNB, tid, l_diag);
+            df_wait(&flags[(j + 1) * ncb + c], abort_word, a.error_word);
+            xt.load(xtile(j + 1), s_l, s_k, kNB, kNB, tid, a.X);
+        }
+#pragma unroll 8
+        for (int kk = 0; kk < kNB; kk += 2) {
+            const int kr = kk + (lane >> 5);
+            acc = df_mfma<LINES_CONTIG>(Ls[kr * kDfLd + wr + (lane & 31)], Xs[kr * kDfLd + wc + (lane & 31)], acc);
+        }
+    }
+
+    // R = B_bc - sum into Xs as R[k = row][line]; X_bc = inv(L_bb) R
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int d = 8 * (r >> 2) + (r & 3);
+        const int row = row_t + (LINES_CONTIG ? d : 0), col = col_t + (LINES_CONTIG ? 0 : d);
+        Xs[row * kDfLd + col] = bv[r] - acc[r];
+    }
+    __syncthreads();
+    f32x16 x = f32x16{};
+#pragma unroll 8
+    for (int kk = 0; kk < kNB; kk += 2) {
+        const int kr = kk + (lane >> 5);
+        x = df_mfma<LINES_CONTIG>(Ds[kr * kLdD + wr + (lane & 31)], Xs[kr * kDfLd + wc + (lane & 31)], x);
+    }
+
+    // publish X_bc (whole tile; rows / lines beyond the matrix are zero), then B = alpha X
+    float* xo = xtile(b);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int d = 8 * (r >> 2) + (r & 3);
+        const int row = row_t + (LINES_CONTIG ? d : 0), col = col_t + (LINES_CONTIG ? 0 : d);
+        xo[row * s_k + col * s_l] = x[r];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&flags[t], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int d = 8 * (r >> 2) + (r & 3);
+        const int dr = LINES_CONTIG ? d : 0, dc = LINES_CONTIG ? 0 : d;
+        if (r0 + row_t + dr < a.nsys && col_t + dc < ncols) pb[dr * a.x_si + dc * a.x_sj] = alpha * x[r];
+    }
+}
+
 // B(i, j) = alpha * X(i, j) through the canonical view of B.
 __global__ void trsm_scatter_kernel(icl_trsm_blocked_args a, float alpha)
 {
@@ -502,6 +704,26 @@ extern "C" hipError_t icl_trsm_step(const icl_trsm_blocked_args* a, int block, h
     return hipGetLastError();
 }
 
+extern "C" int icl_trsm_dataflow_sync_ints(int nsys, int nlines)
+{
+    return ((nsys + kNB - 1) / kNB) * ((nlines + kNB - 1) / kNB) + 2;
+}
+
+extern "C" hipError_t icl_trsm_dataflow(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream)
+{
+    if (a->nsys <= 0 || a->nlines <= 0) return hipSuccess;
+    if (!a->sync || !a->error_word) return hipErrorInvalidValue;
+    const long tiles = (long)((a->nsys + kNB - 1) / kNB) * ((a->nlines + kNB - 1) / kNB);
+    using Kern = void (*)(icl_trsm_blocked_args, float);
+    static const Kern kernels[4] = {trsm_dataflow_kernel<false, false>, trsm_dataflow_kernel<false, true>,
+                                    trsm_dataflow_kernel<true, false>, trsm_dataflow_kernel<true, true>};
+    const bool lines_contig = a->x_sj == 1 || a->x_sj == -1;
+    const bool l_rows_contig = a->l_si == 1 || a->l_si == -1;
+    hipLaunchKernelGGL(kernels[2 * lines_contig + l_rows_contig], dim3((unsigned)tiles), dim3(256), kDfLds, stream, *a,
+                       alpha);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t icl_trsm_scatter(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream)
 {
     if (a->nsys <= 0 || a->nlines <= 0) return hipSuccess;
@@ -514,7 +736,11 @@ extern "C" hipError_t icl_residual_f64(const icl_residual_args* a, hipStream_t s
 {
     if (a->M <= 0 || a->N <= 0) return hipSuccess;
     const long tiles = (long)((a->M + RT - 1) / RT) * ((a->N + RT - 1) / RT);
-    hipLaunchKernelGGL(residual_f64_kernel, dim3((unsigned)tiles), dim3(256), 0, stream, *a);
+    using Kern = void (*)(icl_residual_args);
+    static const Kern kernels[4] = {residual_f64_kernel<false, false>, residual_f64_kernel<false, true>,
+                                    residual_f64_kernel<true, false>, residual_f64_kernel<true, true>};
+    hipLaunchKernelGGL(kernels[2 * (a->p_trans != 0) + (a->q_trans != 0)], dim3((unsigned)tiles), dim3(256), 0, stream,
+                       *a);
     return hipGetLastError();
 }
 

@@ -53,7 +53,7 @@ def _problem(side, uplo, diag, m, n, seed=41, pad=3):
     return A, B
 
 
-@pytest.mark.parametrize("impl", ["strsm_blocked_inv64", "strsm_recursive"])
+@pytest.mark.parametrize("impl", ["strsm_blocked_inv64", "strsm_dataflow64", "strsm_recursive"])
 @pytest.mark.parametrize("v", VARIANTS, ids=vid)
 @pytest.mark.parametrize("shape", [(5, 3), (64, 64), (65, 17), (200, 130), (300, 77)],
                          ids=lambda s: f"{s[0]}x{s[1]}")
