@@ -106,7 +106,6 @@ __global__ void __launch_bounds__(WVM* WVN * 64) cgemm_bf16x3_dma_kernel(CPacked
     const __bf16* a_src = p.Ap + (size_t)m0 * row_elems;
     const __bf16* b_src = p.Bp + (size_t)n0 * row_elems;
     const int lrow = lane >> 3;
-    const int lchunk = (lane & 7) ^ lrow;
 
     auto issue = [&](int ks, int s) {
         unsigned char* stage = ldsc + s * Cfg::STAGE_BYTES;
@@ -115,13 +114,13 @@ __global__ void __launch_bounds__(WVM* WVN * 64) cgemm_bf16x3_dma_kernel(CPacked
 #pragma unroll
         for (int i = 0; i < Cfg::IA; ++i) {
             const int r0 = 8 * (wave + Cfg::NW * i);
-            const __bf16* g = a + (size_t)(r0 + lrow) * row_elems + lchunk * 8;
+            const __bf16* g = a + (size_t)(r0 + lrow) * row_elems + ((lane & 7) ^ lds_swizzle(r0 + lrow)) * 8;
             __builtin_amdgcn_global_load_lds((cglobal_void*)g, (clds_void*)(stage + r0 * Cfg::ROWB), 16, 0, 0);
         }
 #pragma unroll
         for (int i = 0; i < Cfg::IB; ++i) {
             const int r0 = 8 * (wave + Cfg::NW * i);
-            const __bf16* g = b + (size_t)(r0 + lrow) * row_elems + lchunk * 8;
+            const __bf16* g = b + (size_t)(r0 + lrow) * row_elems + ((lane & 7) ^ lds_swizzle(r0 + lrow)) * 8;
             __builtin_amdgcn_global_load_lds((cglobal_void*)g, (clds_void*)(stage + (BM + r0) * Cfg::ROWB), 16, 0,
                                              0);
         }
@@ -155,7 +154,7 @@ __global__ void __launch_bounds__(WVM* WVN * 64) cgemm_bf16x3_dma_kernel(CPacked
                 const unsigned char* row = As + r * Cfg::ROWB;
 #pragma unroll
                 for (int pl = 0; pl < 4; ++pl)
-                    a[pl][i] = *reinterpret_cast<const bf16x8*>(row + (((2 * pl + half) ^ (r & 7)) << 4));
+                    a[pl][i] = *reinterpret_cast<const bf16x8*>(row + (((2 * pl + half) ^ lds_swizzle(r)) << 4));
             }
 #pragma unroll
             for (int j = 0; j < Cfg::FN; ++j) {
@@ -163,7 +162,7 @@ __global__ void __launch_bounds__(WVM* WVN * 64) cgemm_bf16x3_dma_kernel(CPacked
                 const unsigned char* row = Bs + r * Cfg::ROWB;
 #pragma unroll
                 for (int pl = 0; pl < 4; ++pl)
-                    b[pl][j] = *reinterpret_cast<const bf16x8*>(row + (((2 * pl + half) ^ (r & 7)) << 4));
+                    b[pl][j] = *reinterpret_cast<const bf16x8*>(row + (((2 * pl + half) ^ lds_swizzle(r)) << 4));
             }
 #pragma unroll
             for (int j = 0; j < Cfg::FN; ++j) {
@@ -265,7 +264,7 @@ hipError_t cpacked_launch(const icl_gemm_args* g, int opa, int opb, int edge, vo
      OCC}
 
 static const icl_gemm_variant kTable[] = {
-    ICL_CGEMM_BF16X3_PACKED("cgemm_bf16x3d_mfma_256x128w8", 256, 128, 4, 2, 389.5f, 1),
+    ICL_CGEMM_BF16X3_PACKED("cgemm_bf16x3d_mfma_256x128w8", 256, 128, 4, 2, 403.0f, 1),
     ICL_CGEMM_BF16X3_PACKED("cgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, 327.4f, 1),
 };
 ICL_REGISTER_VARIANTS(kTable)

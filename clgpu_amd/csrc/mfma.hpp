@@ -55,6 +55,17 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo)
     lo = (__bf16)(__builtin_isfinite(x) ? r : 0.0f);
 }
 
+// Physical 16-byte chunk slot of logical chunk c of a 128-byte LDS row r is
+// c ^ lds_swizzle(r). A 16-byte LDS read serves 16 lanes per cycle from 64
+// four-byte banks (256 bytes), i.e. from two 128-byte rows' worth of banks; the
+// 16 lanes of a fragment read take 16 consecutive rows at one logical chunk,
+// and (r & 1, c ^ ((r >> 1) & 7)) is distinct over any 16 consecutive rows
+// starting at a multiple of 16, so those reads are free of bank conflicts.
+__device__ __forceinline__ int lds_swizzle(int r)
+{
+    return (r >> 1) & 7;
+}
+
 __device__ __forceinline__ bf16x8 bf16x8_neg(bf16x8 v)
 {
     u32x4 u = __builtin_bit_cast(u32x4, v);

@@ -98,7 +98,8 @@ struct PackedArgs {
 // One DMA instruction writes 64 lanes x 16 bytes = 1 KB contiguous LDS (8 rows of
 // 128 bytes), so rows are unpadded and bank conflicts on the 16-byte fragment
 // reads are avoided by storing logical 16-byte chunk c of row r at physical
-// chunk c ^ (r & 7).
+// chunk c ^ lds_swizzle(r) (the DMA lanes gather the logical chunks from global
+// memory accordingly).
 // ---------------------------------------------------------------------------
 typedef __attribute__((address_space(1))) void global_void;
 typedef __attribute__((address_space(3))) void lds_void;
@@ -136,7 +137,6 @@ __global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_dma_kernel(PackedA
     const __bf16* b_src = p.Bp + (size_t)n0 * (2 * kPackK);
     // lane -> (row within an 8-row DMA block, logical chunk)
     const int lrow = lane >> 3;
-    const int lchunk = (lane & 7) ^ lrow;
 
     auto issue = [&](int ks, int s) {
         unsigned char* stage = ldsd + s * Cfg::STAGE_BYTES;
@@ -145,13 +145,13 @@ __global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_dma_kernel(PackedA
 #pragma unroll
         for (int i = 0; i < Cfg::IA; ++i) {
             const int r0 = 8 * (wave + Cfg::NW * i);
-            const __bf16* g = a + (size_t)(r0 + lrow) * (2 * kPackK) + lchunk * 8;
+            const __bf16* g = a + (size_t)(r0 + lrow) * (2 * kPackK) + ((lane & 7) ^ lds_swizzle(r0 + lrow)) * 8;
             __builtin_amdgcn_global_load_lds((global_void*)g, (lds_void*)(stage + r0 * Cfg::ROWB), 16, 0, 0);
         }
 #pragma unroll
         for (int i = 0; i < Cfg::IB; ++i) {
             const int r0 = 8 * (wave + Cfg::NW * i);
-            const __bf16* g = b + (size_t)(r0 + lrow) * (2 * kPackK) + lchunk * 8;
+            const __bf16* g = b + (size_t)(r0 + lrow) * (2 * kPackK) + ((lane & 7) ^ lds_swizzle(r0 + lrow)) * 8;
             __builtin_amdgcn_global_load_lds((global_void*)g, (lds_void*)(stage + (BM + r0) * Cfg::ROWB), 16, 0, 0);
         }
     };
@@ -180,15 +180,15 @@ __global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_dma_kernel(PackedA
                 for (int i = 0; i < Cfg::FM; ++i) {
                     const int r = wm0 + 32 * i + (lane & 31);
                     const unsigned char* row = As + r * Cfg::ROWB;
-                    ah[i] = *reinterpret_cast<const bf16x8*>(row + ((c ^ (r & 7)) << 4));
-                    al[i] = *reinterpret_cast<const bf16x8*>(row + (((c + 4) ^ (r & 7)) << 4));
+                    ah[i] = *reinterpret_cast<const bf16x8*>(row + ((c ^ lds_swizzle(r)) << 4));
+                    al[i] = *reinterpret_cast<const bf16x8*>(row + (((c + 4) ^ lds_swizzle(r)) << 4));
                 }
 #pragma unroll
                 for (int j = 0; j < Cfg::FN; ++j) {
                     const int r = wn0 + 32 * j + (lane & 31);
                     const unsigned char* row = Bs + r * Cfg::ROWB;
-                    bh[j] = *reinterpret_cast<const bf16x8*>(row + ((c ^ (r & 7)) << 4));
-                    bl[j] = *reinterpret_cast<const bf16x8*>(row + (((c + 4) ^ (r & 7)) << 4));
+                    bh[j] = *reinterpret_cast<const bf16x8*>(row + ((c ^ lds_swizzle(r)) << 4));
+                    bl[j] = *reinterpret_cast<const bf16x8*>(row + (((c + 4) ^ lds_swizzle(r)) << 4));
                 }
 #pragma unroll
                 for (int i = 0; i < Cfg::FM; ++i)
@@ -279,9 +279,9 @@ hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, voi
      OCC}
 
 static const icl_gemm_variant kTable[] = {
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8", 256, 256, 4, 2, 405.8f, 1),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8n", 256, 256, 2, 4, 400.8f, 1),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8", 256, 256, 4, 2, 406.2f, 1),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8n", 256, 256, 2, 4, 417.4f, 1),
     ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, 344.6f, 2),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w4", 256, 256, 2, 2, 400.0f, 1),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w4", 256, 256, 2, 2, 398.1f, 1),
 };
 ICL_REGISTER_VARIANTS(kTable)
