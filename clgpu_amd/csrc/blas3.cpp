@@ -284,7 +284,126 @@ struct Trsm {
     }
 };
 
+// CTRSM: the same recursive blocking on interleaved complex data; strides and
+// offsets count complex elements. The coupling blocks go through the FP32 CGEMM
+// with op N, T or C as in op(A); leaves are complex diagonal-block solves.
+struct CTrsm {
+    iclblasContext* h;
+    bool left, trans, conj, unit, fwd;
+    int m, n;
+    const float* A;
+    long lda;
+    float* B;
+    long ldb;
+    long si, sk; // op(A)(p, q) = A[p * si + q * sk] (conjugated for OP_C)
+
+    const float* sub(int i0, int j0, int* op) const
+    {
+        *op = trans ? (conj ? 2 : 1) : 0;
+        return A + 2 * ((long)i0 * si + (long)j0 * sk);
+    }
+
+    void leaf(int p0, int nb, const float s_post[2]) const
+    {
+        icl_ctrsm_diag_args a{};
+        a.nb = nb;
+        a.unit = unit ? 1 : 0;
+        a.conj = conj ? 1 : 0;
+        a.s_post_re = s_post[0];
+        a.s_post_im = s_post[1];
+        const long e = fwd ? p0 : p0 + nb - 1; // first unknown of the canonical order
+        a.L = A + 2 * e * (si + sk);
+        if (left) {
+            a.nlines = n;
+            a.x_sj = ldb;
+            a.l_si = fwd ? si : -si;
+            a.l_sk = fwd ? sk : -sk;
+            a.X = B + 2 * e;
+            a.x_si = fwd ? 1 : -1;
+        } else {
+            a.nlines = m;
+            a.x_sj = 1;
+            a.l_si = fwd ? sk : -sk;
+            a.l_sk = fwd ? si : -si;
+            a.X = B + 2 * e * ldb;
+            a.x_si = fwd ? ldb : -ldb;
+        }
+        hip_check(icl_ctrsm_diag(&a, h->stream), "ctrsm_diag");
+    }
+
+    // Solve unknowns [p0, p1) (rows of X for left, columns for right).
+    void solve(int p0, int p1, const float s_post[2]) const
+    {
+        const int len = p1 - p0;
+        if (len <= kLeaf) {
+            leaf(p0, len, s_post);
+            return;
+        }
+        const int half = ((len / 2 + kLeaf - 1) / kLeaf) * kLeaf;
+        const int mid = p0 + half;
+        const float minus_one[2] = {-1.0f, 0.0f}, one[2] = {1.0f, 0.0f};
+        int op = 0;
+        if (left) {
+            const int first0 = fwd ? p0 : mid, first1 = fwd ? mid : p1; // solved first
+            const int next0 = fwd ? mid : p0, next1 = fwd ? p1 : mid;
+            solve(first0, first1, s_post);
+            const float* Asub = sub(next0, first0, &op);
+            cgemm_dev(h, ICL_MATH_FP32, op, 0, next1 - next0, n, first1 - first0, minus_one, Asub, lda,
+                      B + 2 * (long)first0, ldb, one, B + 2 * (long)next0, ldb);
+            solve(next0, next1, s_post);
+        } else {
+            const int first0 = fwd ? p0 : mid, first1 = fwd ? mid : p1;
+            const int next0 = fwd ? mid : p0, next1 = fwd ? p1 : mid;
+            solve(first0, first1, s_post);
+            const float* Asub = sub(first0, next0, &op);
+            cgemm_dev(h, ICL_MATH_FP32, 0, op, m, next1 - next0, first1 - first0, minus_one,
+                      B + 2 * (long)first0 * ldb, ldb, Asub, lda, one, B + 2 * (long)next0 * ldb, ldb);
+            solve(next0, next1, s_post);
+        }
+    }
+};
+
 } // namespace
+
+void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, const float alpha[2],
+               const float* A, long lda, float* B, long ldb)
+{
+    if (m <= 0 || n <= 0) return;
+    if (alpha[0] == 0.0f && alpha[1] == 0.0f) {
+        // X = 0 without reading A or B (the documented contract, reference iclBLAS.h:1713 for STRSM)
+        hip_check(icl_caxpby_2d(m, n, 0.0f, 0.0f, nullptr, 0, 0.0f, 0.0f, B, ldb, h->stream), "caxpby");
+        h->last_impl = "ctrsm_zero";
+        return;
+    }
+    CTrsm t{};
+    t.h = h;
+    t.left = side == ICLBLAS_SIDE_LEFT;
+    t.trans = trans != ICLBLAS_OP_N;
+    t.conj = trans == ICLBLAS_OP_C;
+    t.unit = diag == ICLBLAS_DIAG_UNIT;
+    t.m = m;
+    t.n = n;
+    t.A = A;
+    t.lda = lda;
+    t.B = B;
+    t.ldb = ldb;
+    t.si = t.trans ? lda : 1;
+    t.sk = t.trans ? 1 : lda;
+    const bool lower_eff = (uplo == ICLBLAS_FILL_MODE_LOWER) != t.trans; // op(A) lower triangular
+    t.fwd = t.left ? lower_eff : !lower_eff;
+    const int order = t.left ? m : n;
+    if (order <= kLeaf) {
+        // one diagonal block: alpha applied to the solution, as the reference kernel does
+        t.solve(0, order, alpha);
+        h->last_impl = "ctrsm_diag64";
+        return;
+    }
+    const float one[2] = {1.0f, 0.0f};
+    t.solve(0, order, one);
+    if (alpha[0] != 1.0f || alpha[1] != 0.0f)
+        hip_check(icl_caxpby_2d(m, n, alpha[0], alpha[1], B, ldb, 0.0f, 0.0f, B, ldb, h->stream), "caxpby");
+    h->last_impl = "ctrsm_recursive";
+}
 
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
                const float* A, long lda, float* B, long ldb, bool record)

@@ -46,6 +46,10 @@ void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int 
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
                const float* A, long lda, float* B, long ldb, bool record = true);
 
+// Blocked CTRSM on interleaved complex data, in place on B.
+void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, const float alpha[2],
+               const float* A, long lda, float* B, long ldb);
+
 // Host memory mode only (after the stream is synchronised): throws
 // EXECUTION_FAILED, and clears the word, when a dataflow STRSM of this handle
 // aborted a wait.

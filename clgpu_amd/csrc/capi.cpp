@@ -1,5 +1,5 @@
 // C entry points of the hot path: iclblasCreate/Destroy, iclblasSgemm,
-// iclblasCgemm, iclblasStrsm, and the iclBLAS_ext.h extensions.
+// iclblasCgemm, iclblasStrsm, iclblasCtrsm, and the iclBLAS_ext.h extensions.
 //
 // Argument checks run in the reference's order and return before any HIP call
 // (reference: libs/blas/shlib/iclBLAS_S.cpp:487-510, iclBLAS_C.cpp:458-469,
@@ -267,6 +267,40 @@ static iclblasStatus_t strsm_common(iclblasHandle_t handle, iclblasSideMode_t si
         if (host) stage_out<float>(h, B, dB, m, n, ldb);
         finish(h);
         if (host) icl::trsm_check_error(h);
+    });
+}
+
+// reference: libs/blas/shlib/iclBLAS_C.cpp:525-535 (m == 0 or n == 0 -> SUCCESS
+// first, then the STRSM checks).
+ICLBLAS_API iclblasStatus_t iclblasCtrsm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo,
+                                         iclblasOperation_t trans, iclblasDiagType_t diag, int m, int n,
+                                         const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb)
+{
+    bool work = false;
+    const iclblasStatus_t st =
+        trsm_checks(handle, side, uplo, trans, diag, m, n, reinterpret_cast<const float*>(alpha),
+                    reinterpret_cast<const float*>(A), lda, reinterpret_cast<const float*>(B), ldb, &work);
+    if (!work) return st;
+    return guarded([&] {
+        HandleLock lock(handle);
+        iclblasContext* h = handle;
+        float a[2];
+        std::memcpy(a, alpha, sizeof(a));
+        typedef float cf[2];
+        const int order = side == ICLBLAS_SIDE_LEFT ? m : n;
+        const bool nonzero = a[0] != 0.0f || a[1] != 0.0f;
+        const float* dA = reinterpret_cast<const float*>(A);
+        float* dB = reinterpret_cast<float*>(B);
+        const bool host = h->memory_mode == ICLBLAS_MEMORY_HOST;
+        if (host) {
+            dA = nonzero ? reinterpret_cast<const float*>(
+                               stage_in<cf>(h, kSlotA, reinterpret_cast<const cf*>(A), order, order, lda, true))
+                         : nullptr;
+            dB = reinterpret_cast<float*>(stage_in<cf>(h, kSlotB, reinterpret_cast<const cf*>(B), m, n, ldb, nonzero));
+        }
+        icl::ctrsm_dev(h, side, uplo, trans, diag, m, n, a, dA, lda, dB, ldb);
+        if (host) stage_out<cf>(h, reinterpret_cast<cf*>(B), reinterpret_cast<const cf*>(dB), m, n, ldb);
+        finish(h);
     });
 }
 

@@ -78,6 +78,23 @@ typedef struct icl_trsm_diag_args {
 } icl_trsm_diag_args;
 hipError_t icl_trsm_diag(const icl_trsm_diag_args* a, hipStream_t stream);
 
+/* Complex version of the diagonal-block solve: L and X hold interleaved
+ * {re, im} pairs and the strides count complex elements. conj: every element
+ * of L read is conjugated (op = conjugate transpose). X = s_post * (L^-1 X0),
+ * with the division and products of the reference's complex helpers. */
+typedef struct icl_ctrsm_diag_args {
+    int nb;
+    int nlines;
+    const float* L;
+    long l_si, l_sk;
+    float* X;
+    long x_si, x_sj;
+    int unit;
+    int conj;
+    float s_post_re, s_post_im;
+} icl_ctrsm_diag_args;
+hipError_t icl_ctrsm_diag(const icl_ctrsm_diag_args* a, hipStream_t stream);
+
 /* Blocked solve of the canonical lower-triangular system L X = B with nsys
  * unknowns and nlines right-hand sides: L(i, k) = L[i * l_si + k * l_sk],
  * B(i, j) = B[i * x_si + j * x_sj] (strides may be negative). The 64 x 64

@@ -98,6 +98,21 @@ def triangular(order: int, lower: bool = True, ld: int | None = None, name: str 
     return np.asfortranarray(out.astype(np.float32))
 
 
+def complex_triangular(order: int, lower: bool = True, ld: int | None = None, name: str = "A", seed: int = 0,
+                       poison: bool = True) -> np.ndarray:
+    """complex64 triangular test matrix: the real parts are triangular(...)'s
+    values (diagonal sign * U[1, 2], stored strict triangle U[-1, 1] / sqrt(order)),
+    the imaginary parts of the stored triangle are an independent draw of the same
+    kind scaled like the off-diagonal (so the diagonal dominates); the other
+    triangle is NaN + NaN i when `poison`, 0 otherwise."""
+    ld = order if ld is None else ld
+    re = triangular(order, lower, ld, name, seed, poison).astype(np.float64)
+    im = triangular(order, lower, ld, name, seed + 7919, poison).astype(np.float64)
+    d = np.arange(order)
+    im[d, d] = 0.5 * im[d, d] / np.sqrt(order)
+    return np.asfortranarray((re + 1j * im).astype(np.complex64))
+
+
 def quarter_integers(count: int, seed: int = 0, stream: int = 0) -> np.ndarray:
     """Values in {-1, -0.75, ..., 1} (uniform over the 9 values): the data family
     of the reference's implementation-level tests (tests/blas_funcs/test_helpers.hpp:120-127),

@@ -2,7 +2,8 @@
 
 Names, enum values and argument order are those of the reference's
 include/iclBLAS/iclBLAS.h (iclblasCreate :172, iclblasDestroy :179,
-iclblasSgemm :1685, iclblasStrsm :1719, iclblasCgemm :1799) plus the
+iclblasSgemm :1685, iclblasStrsm :1719, iclblasCgemm :1799, iclblasCtrsm
+:1925) plus the
 extensions of include/iclBLAS_ext.h. Every function returns the C status;
 `check()` turns a non-success status into IclblasError.
 
@@ -71,6 +72,7 @@ _SIGS = {
     "iclblasSgemm": [_P, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _FP, _P, _I],
     "iclblasCgemm": [_P, _I, _I, _I, _I, _I, _CP, _P, _I, _P, _I, _CP, _P, _I],
     "iclblasStrsm": [_P, _I, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I],
+    "iclblasCtrsm": [_P, _I, _I, _I, _I, _I, _I, _CP, _P, _I, _P, _I],
     "iclblasStrsmRefined": [_P, _I, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _I],
     "iclblasSgemmBackward": [_P, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _P, _I, _P, _I, _P, _I],
     "iclblasStrsmBackward": [_P, _I, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _P, _I, _P, _I, _P, _I],
@@ -168,6 +170,10 @@ def iclblasCgemm(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C
 
 def iclblasStrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
     return load().iclblasStrsm(handle, side, uplo, trans, diag, m, n, _fscalar(alpha), _ptr(A), lda, _ptr(B), ldb)
+
+
+def iclblasCtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
+    return load().iclblasCtrsm(handle, side, uplo, trans, diag, m, n, _cscalar(alpha), _ptr(A), lda, _ptr(B), ldb)
 
 
 def iclblasStrsmRefined(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, max_sweeps):

@@ -6,7 +6,8 @@ bench.py's CPU-baseline leg. The library (clgpu_amd) never imports it.
 liboracle.so (oracle.c) restates the reference's CPU semantics:
   sgemm/cgemm  <- cpu_gemm, reference tests/blas_funcs/test_gemm.hpp:83-201
   strsm        <- Strsm_naive, reference libs/blas/funcs/Strsm/Strsm_naive.cl:18-126
-and provides FP64 references (dgemm, zgemm, dtrsm) for error reporting.
+  ctrsm        <- Ctrsm_naive, reference libs/blas/funcs/Ctrsm/Ctrsm_naive.cl:16-183
+and provides FP64 references (dgemm, zgemm, dtrsm, ztrsm) for error reporting.
 _ref/libref_gemm.so, when present, is the reference's own cpu_gemm compiled from
 its headers (oracle/Makefile, target `ref`).
 
@@ -54,6 +55,8 @@ def lib():
         L.orc_cgemm.argtypes = [I, I, I, I, I, P, P, I, P, I, P, P, I]
         L.orc_strsm.argtypes = [I, I, I, I, I, I, F, P, I, P, I]
         L.orc_dtrsm.argtypes = [I, I, I, I, I, I, D, P, I, P, I]
+        L.orc_ctrsm.argtypes = [I, I, I, I, I, I, P, P, I, P, I]
+        L.orc_ztrsm.argtypes = [I, I, I, I, I, I, P, P, I, P, I]
         L.orc_dgemm.argtypes = [I, I, I, I, I, D, P, I, P, I, D, P, I, P, P]
         L.orc_dgemm_sample.argtypes = [I, I, I, D, P, I, P, I, D, P, I, I, P, P, P, P]
         L.orc_zgemm.argtypes = [I, I, I, I, I, P, P, I, P, I, P, P, I, P, P]
@@ -124,6 +127,14 @@ def strsm(side, uplo, trans, diag, m, n, alpha, A, B):
     return B
 
 
+def ctrsm(side, uplo, trans, diag, m, n, alpha, A, B):
+    """Ctrsm_naive restated (complex64 in place on B)."""
+    _chk(A, np.complex64), _chk(B, np.complex64)
+    al = np.array([complex(alpha).real, complex(alpha).imag], np.float32)
+    lib().orc_ctrsm(side, uplo, trans, diag, m, n, _p(al), _p(A), _ld(A), _p(B), _ld(B))
+    return B
+
+
 # ---------------------------------------------------------------- layer 2
 def dgemm(transa, transb, m, n, k, alpha, A, B, beta=0.0, C=None):
     """Returns (C64, absAB) as dense m x n float64 (C order irrelevant: Fortran view)."""
@@ -179,6 +190,15 @@ def dtrsm(side, uplo, trans, diag, m, n, alpha, A, B):
     _chk(A, np.float32)
     X = np.asfortranarray(np.array(B[:m, :n], dtype=np.float64))
     lib().orc_dtrsm(side, uplo, trans, diag, m, n, float(alpha), _p(A), _ld(A), _p(X), _ld(X))
+    return X
+
+
+def ztrsm(side, uplo, trans, diag, m, n, alpha, A, B):
+    """Complex FP64 substitution; B (complex64) is read, a complex128 m x n result returned."""
+    _chk(A, np.complex64)
+    X = np.asfortranarray(np.array(B[:m, :n], dtype=np.complex128))
+    al = np.array([complex(alpha).real, complex(alpha).imag], np.float64)
+    lib().orc_ztrsm(side, uplo, trans, diag, m, n, _p(al), _p(A), _ld(A), _p(X), _ld(X))
     return X
 
 

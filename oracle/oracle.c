@@ -19,13 +19,16 @@
  *       (FP_CONTRACT ON), and the reference's own known answer
  *       3x4_right_up_trans_ndiag (tests/iclBLAS/test_Strsm.cpp:185-225) is only
  *       met within its 4-ULP tolerance with the fused form.
+ *     - orc_ctrsm: the substitution of libs/blas/funcs/Ctrsm/Ctrsm_naive.cl:16-183
+ *       with the fma-based cmul / cdiv of core/src/ocl/complex.h.cl:32-39,
+ *       OP_C conjugating A, alpha applied after the solve.
  *  2. FP64 references used for error reporting: orc_dgemm / orc_zgemm (full
  *     double arithmetic, plus the |op(A)||op(B)| product for the elementwise
- *     bound), orc_dtrsm (the same substitution in double), sampled variants
+ *     bound), orc_dtrsm / orc_ztrsm (the same substitutions in double), sampled variants
  *     for large shapes, and a column-parallel orc_sgemm_mt for the CPU baseline.
  *
  * Compiled with -ffp-contract=off so that no product is fused implicitly; the
- * only FMAs are the explicit ones of the STRSM elimination.
+ * only FMAs are the explicit ones of the TRSM eliminations and complex helpers.
  */
 #include <math.h>
 #include <pthread.h>
@@ -193,6 +196,65 @@ void orc_dtrsm(int side, int uplo, int trans, int diag, int m, int n, double alp
                double* b, int ldb)
 {
     TRSM_BODY(double, fma)
+}
+
+/* ------------------------------------------------------------------------- */
+/* Restatement of Ctrsm_naive (libs/blas/funcs/Ctrsm/Ctrsm_naive.cl:16-183)    */
+/* with the complex helpers of core/src/ocl/complex.h.cl:32-39:                */
+/*   cmul(a, b) = (fma(ax, bx, -ay*by), fma(ax, by, ay*bx))                    */
+/*   cdiv(a, b) = (fma(ax, bx, ay*by) / d, fma(ay, bx, -ax*by) / d),           */
+/*                d = fma(bx, bx, by*by)                                       */
+/* The eight loop nests of the kernel reduce to two shapes (as for the real    */
+/* kernel): left solves run forward when lower == (trans == N) and use         */
+/* a(k, i) for N, a(i, k) otherwise; right solves run backward when            */
+/* lower == (trans == N) and use a(i, k) for N, a(k, i) otherwise. OP_C        */
+/* conjugates every element of A read, the diagonal included. B is updated by  */
+/* `b -= cmul(coef, x)` and the solved entry is stored as cmul(x, alpha).      */
+/* ------------------------------------------------------------------------- */
+#define CTRSM_BODY(T, FMA)                                                                                              \
+    const int left = side == 0, lower = uplo == 1, ntrans = trans == 0, cj = trans == 2, ndiag = diag == 0;          \
+    const int order = left ? m : n, lines = left ? n : m;                                                            \
+    const int fwd = left ? (lower == ntrans) : !(lower == ntrans);                                                   \
+    for (int t = 0; t < order; t++) {                                                                                \
+        const int i = fwd ? t : order - 1 - t;                                                                       \
+        for (int j = 0; j < lines; j++) {                                                                            \
+            T *xb = left ? &b[2 * ((long)j * ldb + i)] : &b[2 * ((long)i * ldb + j)];                                \
+            T xr = xb[0], xi = xb[1];                                                                                \
+            if (ndiag) {                                                                                             \
+                const T dr = (T)a[2 * ((long)i * lda + i)], di = (cj ? -1 : 1) * (T)a[2 * ((long)i * lda + i) + 1];  \
+                const T dv = FMA(dr, dr, di * di);                                                                   \
+                const T qr = FMA(xr, dr, xi * di) / dv, qi = FMA(xi, dr, -xr * di) / dv;                             \
+                xr = qr;                                                                                             \
+                xi = qi;                                                                                             \
+            }                                                                                                        \
+            const int k0 = fwd ? i + 1 : 0, k1 = fwd ? order : i;                                                    \
+            for (int k = k0; k < k1; k++) {                                                                          \
+                /* left: coef = N ? a(k, i) : a(i, k); right: coef = N ? a(i, k) : a(k, i) */                       \
+                const int use_ki = left ? ntrans : !ntrans;                                                          \
+                const long ia = 2 * (use_ki ? (long)i * lda + k : (long)k * lda + i);                                \
+                const T cr = (T)a[ia], ci = (cj ? -1 : 1) * (T)a[ia + 1];                                            \
+                const T tr = FMA(cr, xr, -ci * xi), ti = FMA(cr, xi, ci * xr);                                       \
+                T *bk = left ? &b[2 * ((long)j * ldb + k)] : &b[2 * ((long)k * ldb + j)];                            \
+                bk[0] -= tr;                                                                                         \
+                bk[1] -= ti;                                                                                         \
+            }                                                                                                        \
+            xb[0] = FMA(xr, (T)alpha[0], -xi * (T)alpha[1]);                                                         \
+            xb[1] = FMA(xr, (T)alpha[1], xi * (T)alpha[0]);                                                          \
+        }                                                                                                            \
+    }
+
+/* a, b: interleaved complex64; alpha {re, im} */
+void orc_ctrsm(int side, int uplo, int trans, int diag, int m, int n, const float* alpha, const float* a, int lda,
+               float* b, int ldb)
+{
+    CTRSM_BODY(float, fmaf)
+}
+
+/* Same substitution in complex double: a as stored (complex64), b complex128 in/out. */
+void orc_ztrsm(int side, int uplo, int trans, int diag, int m, int n, const double* alpha, const float* a, int lda,
+               double* b, int ldb)
+{
+    CTRSM_BODY(double, fma)
 }
 
 /* ------------------------------------------------------------------------- */

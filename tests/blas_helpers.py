@@ -70,3 +70,16 @@ def bound_ratio(err, k, absAB, extra=None):
     with np.errstate(divide="ignore", invalid="ignore"):
         r = np.where(den > 0, err / den, np.where(err == 0, 0.0, np.inf))
     return float(r.max(initial=0.0))
+
+
+def complex_near(expected, result, eps=4e-4):
+    """EXPECT_COMPLEX_NEAR of the reference (tests/iclBLAS/test_Ctrsm.cpp:21-34):
+    per component, |1 - result/expected| <= eps, or |result| <= eps where the
+    expected component is 0."""
+    e = np.asarray(expected, np.complex128)
+    r = np.asarray(result, np.complex128)
+    ok = True
+    for ev, rv in ((e.real, r.real), (e.imag, r.imag)):
+        nz = ev != 0
+        ok &= bool(np.all(np.abs(1.0 - rv[nz] / ev[nz]) <= eps)) and bool(np.all(np.abs(rv[~nz]) <= eps))
+    return ok

@@ -5,6 +5,7 @@ Sources (reference tree, read as text, nothing executed):
   tests/iclBLAS/test_Sgemm.cpp:18-287   6 SGEMM cases
   tests/iclBLAS/test_Cgemm.cpp:19-431   8 CGEMM cases
   tests/iclBLAS/test_Strsm.cpp:23-334   8 STRSM cases
+  tests/iclBLAS/test_Ctrsm.cpp:36-637  12 CTRSM cases
 
 Inputs and literal expected arrays are copied value for value. Expected
 values the reference writes as float expressions (STRSM, e.g. `1.1f*2.f/3.f`)
@@ -174,13 +175,79 @@ def strsm_cases():
     return cases
 
 
+def _pairs(lo, hi):
+    """(lo, lo+1), (lo+2, lo+3), ... up to hi: the reference's running b arrays."""
+    return [(v, v + 1) for v in range(lo, hi, 2)]
+
+
+def ctrsm_cases():
+    """tests/iclBLAS/test_Ctrsm.cpp:36-637. The reference computes each expected
+    array with an inline std::complex<float> substitution and compares with
+    EXPECT_COMPLEX_NEAR (relative 4e-4 per component, absolute for zero
+    components, :21-34). Here the expected values are the exact solution of the
+    same system, op(A) X = alpha B (left) or X op(A) = alpha B (right), solved in
+    complex128 by numpy from the mathematical definition (triangle of A, unit
+    diagonal, conjugation), independent of the oracle's substitution order; the
+    padding rows of B (ldb > m) keep their input values."""
+    A1 = [(1, 2), (-1, -2), (-3, -4), (3, 4), (5, 6), (-5, -6), (7, 8), (9, 10), (11, 12)]
+    A16 = [(1, 2), (-1, -2), (-3, -4), (-5, -6), (3, 4), (5, 6), (-7, -8), (-9, -10), (7, 8), (9, 10), (11, 12),
+           (-11, -12), (13, 14), (15, 16), (17, 18), (19, 20)]
+    A4 = [(1, 2), (3, 4), (5, 6), (-11, -12), (-1, -2), (7, 8), (9, 10), (-11, -12), (-3, -4), (-5, -6), (11, 12),
+          (-11, -12)]
+    AL = [(1, 2), (3, 4), (5, 6), (-1, -2), (7, 8), (9, 10), (-3, -4), (-5, -6), (11, 12)]
+    A20 = [(1, 2), (3, 4), (5, 6), (7, 8), (-1, -2), (-3, -4), (9, 10), (11, 12), (13, 14), (-5, -6), (-7, -8),
+           (-9, -10), (15, 16), (17, 18), (-11, -12), (-13, -14), (-15, -16), (-17, -18), (19, 20), (-19, -20)]
+    B5 = [(13, 14), (15, 16), (17, 18), (-1, -2), (19, 20), (21, 22), (23, 24), (-3, -4), (25, 26), (27, 28),
+          (29, 30), (-5, -6), (31, 32), (33, 34), (35, 36), (-7, -8)]
+    B6 = [(21, 22), (23, 24), (25, 26), (27, 28), (-1, -2), (29, 30), (31, 32), (33, 34), (35, 36), (-3, -4),
+          (37, 38), (39, 40), (41, 42), (43, 44), (-5, -6)]
+    B11 = [(21, 22), (23, 24), (25, 26), (-1, -2), (-3, -4), (27, 28), (29, 30), (31, 32), (-5, -6), (-7, -8),
+           (33, 34), (35, 36), (37, 38), (-9, -10), (-11, -12), (39, 40), (41, 42), (43, 44), (-13, -14), (-15, -16)]
+    raw = [
+        ("3x3_left_up_ntrans_ndiag", 0, 0, 0, 0, 3, 3, 3, 3, (1.1, 2.3), A1, _pairs(13, 31)),
+        ("4x3_left_up_trans_diag", 0, 0, 1, 1, 4, 3, 4, 4, (-1.1, 2.3), A16, _pairs(21, 45)),
+        ("3x4_left_up_conj_ndiag", 0, 0, 2, 0, 3, 4, 3, 3, (-1.1, -2.3), A1, _pairs(13, 37)),
+        ("3x3_left_low_ntrans_diag_lda4", 0, 1, 0, 1, 3, 3, 4, 3, (1.1, -2.3), A4, _pairs(13, 31)),
+        ("3x4_left_low_trans_ndiag_ldb4", 0, 1, 1, 0, 3, 4, 3, 4, (2.1, 1.3), AL, B5),
+        ("4x3_left_low_conj_diag_lds5", 0, 1, 2, 1, 4, 3, 5, 5, (-2.1, 1.3), A20, B6),
+        ("3x3_right_up_ntrans_ndiag", 1, 0, 0, 0, 3, 3, 3, 3, (-2.1, -1.3), A1, _pairs(13, 31)),
+        ("4x3_right_up_trans_ndiag_ldb5", 1, 0, 1, 0, 4, 3, 3, 4, (2.1, -1.3), A1, _pairs(13, 37)),
+        ("3x4_right_up_conj_diag_lda5", 1, 0, 2, 1, 3, 4, 4, 3, (1.3, 2.3), A16, _pairs(21, 45)),
+        ("3x3_right_low_ntrans_diag", 1, 1, 0, 1, 3, 3, 3, 3, (-1.3, -2.3), AL, _pairs(13, 31)),
+        ("3x4_right_low_trans_ndiag_lds5", 1, 1, 1, 0, 3, 4, 5, 5, (1.3, -2.3), A20, B11),
+        ("4x3_right_low_conj_ndiag", 1, 1, 2, 0, 4, 3, 3, 4, (1.3, 2.3), AL, _pairs(13, 37)),
+    ]
+    out = []
+    for name, side, uplo, trans, diag, m, n, lda, ldb, alpha, A, B in raw:
+        order = m if side == 0 else n
+        assert len(A) == lda * order and len(B) == ldb * n, name
+        al = complex(f(alpha[0]), f(alpha[1]))
+        Am = np.array([complex(f(a), f(b)) for a, b in A]).reshape(order, lda).T  # column-major
+        Bm = np.array([complex(f(a), f(b)) for a, b in B]).reshape(n, ldb).T
+        T = Am[:order, :order].copy()
+        T = np.tril(T) if uplo == 1 else np.triu(T)
+        if diag == 1:
+            np.fill_diagonal(T, 1.0)
+        opT = T if trans == 0 else (T.T if trans == 1 else T.conj().T)
+        rhs = al * Bm[:m, :n]
+        X = np.linalg.solve(opT, rhs) if side == 0 else np.linalg.solve(opT.T, rhs.T).T
+        E = Bm.copy()
+        E[:m, :n] = X
+        out.append(dict(name=name, side=side, uplo=uplo, trans=trans, diag=diag, m=m, n=n, lda=lda, ldb=ldb,
+                        alpha=[float(al.real), float(al.imag)], A=cplx(A), B=cplx(B),
+                        expected=[[float(z.real), float(z.imag)] for z in E.T.reshape(-1)],
+                        compare="complex_near_4e-4"))
+    return out
+
+
 def main():
-    data = {"source": "intel/clGPU tests/iclBLAS/test_{Sgemm,Cgemm,Strsm}.cpp",
-            "sgemm": sgemm_cases(), "cgemm": cgemm_cases(), "strsm": strsm_cases()}
+    data = {"source": "intel/clGPU tests/iclBLAS/test_{Sgemm,Cgemm,Strsm,Ctrsm}.cpp",
+            "sgemm": sgemm_cases(), "cgemm": cgemm_cases(), "strsm": strsm_cases(), "ctrsm": ctrsm_cases()}
     out = Path(__file__).with_name("kats.json")
     out.write_text(json.dumps(data, indent=None, separators=(",", ":")))
     print(f"wrote {out} ({out.stat().st_size} bytes): "
-          f"{len(data['sgemm'])} sgemm, {len(data['cgemm'])} cgemm, {len(data['strsm'])} strsm")
+          f"{len(data['sgemm'])} sgemm, {len(data['cgemm'])} cgemm, {len(data['strsm'])} strsm, "
+          f"{len(data['ctrsm'])} ctrsm")
 
 
 if __name__ == "__main__":

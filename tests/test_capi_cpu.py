@@ -79,7 +79,7 @@ def test_sgemm_status_order(forged):
     assert _sgemm(bad_handle) == INV
 
 
-def test_cgemm_and_strsm_status_order(forged):
+def test_cgemm_and_trsm_status_order(forged):
     bad_handle, _keep = forged
     z = np.zeros((4, 4), np.complex64, order="F")
     f = np.zeros((4, 4), np.float32, order="F")
@@ -97,6 +97,13 @@ def test_cgemm_and_strsm_status_order(forged):
     assert ib.iclblasStrsm(None, 3, 0, 0, 0, 2, 2, 1.0, f, 4, f, 4) == INV  # side out of range
     assert ib.iclblasStrsm(bad_handle, 0, 0, 0, 0, 2, 2, 1.0, f, 4, f, 4) == INV
     assert ib.iclblasStrsmRefined(None, 0, 0, 0, 0, 2, 2, 1.0, f, 4, f, 4, 1) == INV
+    # CTRSM: the same chain (iclBLAS_C.cpp:525-535 returns SUCCESS for m == 0 or n == 0 first)
+    assert ib.iclblasCtrsm(None, 0, 0, 0, 0, 2, 0, None, None, 0, None, 0) == OK
+    assert ib.iclblasCtrsm(None, 0, 0, 0, 0, -1, 2, 1, z, 4, z, 4) == INV
+    assert ib.iclblasCtrsm(None, 0, 0, 2, 0, 2, 2, 1, z, 4, None, 4) == INV
+    assert ib.iclblasCtrsm(None, 1, 0, 2, 0, 2, 5, 1, z, 4, z, 4) == INV  # lda < n for a right-side solve
+    assert ib.iclblasCtrsm(None, 0, 0, 3, 0, 2, 2, 1, z, 4, z, 4) == INV  # trans out of range
+    assert ib.iclblasCtrsm(bad_handle, 0, 0, 2, 1, 2, 2, 1, z, 4, z, 4) == INV
 
 
 def test_handle_entry_points_reject_invalid_handles(forged):
@@ -116,9 +123,9 @@ def test_unimplemented_entry_points_report_not_supported():
     f = L.iclblasSaxpy
     f.restype = ct.c_int
     assert f(None, 0, None, None, 1, None, 1) == ib.STATUS_NOT_SUPPORTED
-    g = L.iclblasCtrsm
+    g = L.iclblasCtrmm
     g.restype = ct.c_int
-    assert g(None, 0, 0, 0, 0, 1, 1, None, None, 1, None, 1) == ib.STATUS_NOT_SUPPORTED
+    assert g(None, 0, 0, 0, 0, 1, 1, None, None, 1, None, 1, None, 1) == ib.STATUS_NOT_SUPPORTED
 
 
 def test_create_without_a_gpu_reports_not_initialized():

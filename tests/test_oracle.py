@@ -18,7 +18,7 @@ import pytest
 import scipy.linalg as sla
 
 import oracle
-from blas_helpers import adaptive_ok, cmat, complex_eq, flat, float_eq, fmat
+from blas_helpers import adaptive_ok, cmat, complex_eq, complex_near, flat, float_eq, fmat
 from clgpu_amd import datagen
 
 KATS = json.loads((Path(__file__).parent / "golden" / "kats.json").read_text())
@@ -210,3 +210,77 @@ def test_planted_trsm_order_mistake_is_caught():
     X_ref = sla.solve_triangular(Af, B.astype(np.float64), lower=False)
     wrong = oracle.dtrsm(0, 1, 0, 0, m, n, 1.0, np.asfortranarray(np.nan_to_num(A)), B)  # solves as lower
     assert not np.allclose(wrong, X_ref, rtol=1e-6)
+
+
+# ------------------------------------------------------------------ CTRSM
+@pytest.mark.parametrize("case", KATS["ctrsm"], ids=lambda c: c["name"])
+def test_oracle_ctrsm_kat(case):
+    c = case
+    order = c["m"] if c["side"] == 0 else c["n"]
+    A = cmat(c["A"], c["lda"], order)
+    B = cmat(c["B"], c["ldb"], c["n"])
+    oracle.ctrsm(c["side"], c["uplo"], c["trans"], c["diag"], c["m"], c["n"], complex(*c["alpha"]), A, B)
+    exp = np.asarray(c["expected"], np.float64)
+    assert complex_near(exp[:, 0] + 1j * exp[:, 1], flat(B))
+
+
+CVARIANTS = [(s, u, t, d) for s in (0, 1) for u in (0, 1) for t in (0, 1, 2) for d in (0, 1)]
+
+
+def _scipy_ctrsm(side, uplo, trans, diag, m, n, alpha, A, B):
+    order = m if side == 0 else n
+    Ac = np.nan_to_num(A[:order, :order].astype(np.complex128), nan=0.0)
+    Ac = np.tril(Ac) if uplo == 1 else np.triu(Ac)
+    if diag == 1:
+        np.fill_diagonal(Ac, 1.0)
+    opA = Ac if trans == 0 else (Ac.T if trans == 1 else Ac.conj().T)
+    lower_eff = (uplo == 1) != (trans != 0)
+    rhs = alpha * B[:m, :n].astype(np.complex128)
+    if side == 0:
+        return sla.solve_triangular(opA, rhs, lower=lower_eff)
+    return sla.solve_triangular(opA.T, rhs.T, lower=not lower_eff).T
+
+
+@pytest.mark.parametrize("side,uplo,trans,diag", CVARIANTS)
+def test_oracle_ctrsm_solves_and_reads_only_its_triangle(side, uplo, trans, diag):
+    """The complex float restatement and the complex FP64 substitution solve
+    op(A) X = alpha B (checked with scipy's complex solver; op = A^H conjugates)
+    while the unused triangle -- and, for a unit diagonal, the diagonal -- hold
+    NaN and are never read."""
+    m, n = 19, 13
+    order = m if side == 0 else n
+    A = datagen.complex_triangular(order, lower=bool(uplo), ld=order + 1, seed=11, poison=True)
+    if diag == 1:
+        A[np.arange(order), np.arange(order)] = np.nan
+    B = datagen.complex_matrix(m + 2, n, name="B", seed=11)
+    alpha = 0.8 - 0.35j
+    X_ref = _scipy_ctrsm(side, uplo, trans, diag, m, n, alpha, A, B)
+    Z = oracle.ztrsm(side, uplo, trans, diag, m, n, alpha, A, B)
+    assert np.all(np.isfinite(Z))
+    assert np.allclose(Z, X_ref, rtol=1e-12, atol=1e-12)
+    X = B.copy(order="F")
+    oracle.ctrsm(side, uplo, trans, diag, m, n, alpha, A, X)
+    assert np.array_equal(X[m:], B[m:])  # padding rows untouched
+    assert np.all(np.isfinite(X[:m]))
+    assert np.max(np.abs(X[:m] - X_ref)) <= 1e-5 * max(1.0, np.max(np.abs(X_ref)))
+
+
+def test_oracle_ctrsm_identity_gives_alpha_b():
+    m, n = 7, 5
+    A = np.asfortranarray(np.eye(m, dtype=np.complex64))
+    B = datagen.complex_matrix(m, n, name="B")
+    X = B.copy(order="F")
+    oracle.ctrsm(0, 1, 2, 0, m, n, 1.5 - 0.5j, A, X)
+    assert np.allclose(X, B.astype(np.complex128) * (1.5 - 0.5j), rtol=1e-6, atol=1e-6)
+
+
+def test_planted_ctrsm_conjugation_mistake_is_caught():
+    """A plausible mistake -- OP_C solved as plain transpose (conjugate dropped)
+    -- must fail the scipy check used above."""
+    m, n = 10, 4
+    A = datagen.complex_triangular(m, lower=True, seed=5, poison=False)
+    B = datagen.complex_matrix(m, n, name="B", seed=5)
+    X_ref = _scipy_ctrsm(0, 1, 2, 0, m, n, 1.0, A, B)
+    wrong = oracle.ztrsm(0, 1, 1, 0, m, n, 1.0, A, B)  # conjugate dropped
+    assert not np.allclose(wrong, X_ref, rtol=1e-6)
+    assert np.allclose(oracle.ztrsm(0, 1, 2, 0, m, n, 1.0, A, B), X_ref, rtol=1e-12, atol=1e-12)
