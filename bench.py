@@ -45,8 +45,9 @@ def parse():
     ap.add_argument("--trans", default="NN", help="NN, NT, TN or TT for the timed op")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check (profiling runs)")
     ap.add_argument("--no-extra", action="store_true", help="skip the CGEMM / STRSM rows (same as --rows sgemm)")
-    ap.add_argument("--rows", default="sgemm,cgemm,strsm",
-                    help="comma list of workloads to run: sgemm (the headline line), cgemm, strsm")
+    ap.add_argument("--rows", default="sgemm,sgemm_more,cgemm,strsm,ctrsm,chain",
+                    help="comma list of workloads to run: sgemm (the headline line), sgemm_more (the other SGEMM "
+                         "configurations), cgemm, strsm, ctrsm, chain (the differentiable GEMM+TRSM chain)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy kernel first (scales HBM byte counters)")
     return ap.parse_args()
@@ -89,6 +90,151 @@ def _time(stream, fn, steps: int, warmup: int) -> float:
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def sgemm_rows(h, ib, stream, steps: int) -> dict:
+    """The other SGEMM configurations: 8192^3 NT / TN / TT, NN m=4096 n=8192
+    k=2048 with lda=ldb=ldc=4160, and NN 256^3 with beta = 0.5, each on both
+    paths, with the error against an FP64 product computed with torch on the GPU."""
+    import torch
+
+    from clgpu_amd import datagen
+    dev = torch.device("cuda:0")
+    out = {}
+
+    def run(name, ta, tb, m, n, k, lda, ldb, ldc, beta):
+        # column-major (ld x cols) storage as torch (cols, ld)
+        acols, bcols = (k if ta == 0 else m), (n if tb == 0 else k)
+        A = torch.from_numpy(np.ascontiguousarray(datagen.matrix(lda, acols, name="A", seed=1).T)).to(dev)
+        B = torch.from_numpy(np.ascontiguousarray(datagen.matrix(ldb, bcols, name="B", seed=1).T)).to(dev)
+        C0 = torch.from_numpy(np.ascontiguousarray(datagen.matrix(ldc, n, name="C", seed=1).T)).to(dev)
+        # storage X_t[c, r] = X(r, c); op(A) as (m, k), op(B) as (k, n)
+        opA = A.double()[:k, :m].t() if ta == 0 else A.double()[:m, :k]
+        opB = B.double()[:n, :k].t() if tb == 0 else B.double()[:k, :n]
+        ref = (opA @ opB + beta * C0.double()[:, :m].t()).t()  # storage layout (n, m)
+        bound = (k * 2.0 ** -24 * (opA.abs() @ opB.abs() + abs(beta) * C0.double()[:, :m].t().abs())).t()
+        res = {}
+        for path, mode in (("fp32", ib.MATH_FP32), ("bf16x3", ib.MATH_SPLIT_BF16X3)):
+            h.set_math_mode(mode)
+            h.set_implementation(None)
+            C = C0.clone()
+            ib.check(ib.iclblasSgemm(h.value, ta, tb, m, n, k, 1.0, A.data_ptr(), lda, B.data_ptr(), ldb, beta,
+                                     C.data_ptr(), ldc), "iclblasSgemm")
+            torch.cuda.synchronize()
+            err = (C.double()[:, :m] - ref).abs()
+            pad_ok = bool(torch.equal(C[:, m:], C0[:, m:]))
+            Cw = C0.clone()
+            ms = _time(stream, lambda: ib.check(ib.iclblasSgemm(h.value, ta, tb, m, n, k, 1.0, A.data_ptr(), lda,
+                                                                B.data_ptr(), ldb, beta, Cw.data_ptr(), ldc)),
+                       steps, 3)
+            tf = 2.0 * m * n * k / (ms * 1e-3) / 1e12
+            res[path] = {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
+                         "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA,
+                         "max_abs_err": float(err.max()),
+                         "max_err_over_bound_unit": float((err / bound.clamp_min(1e-300)).max()),
+                         "padding_untouched": pad_ok}
+            del C, Cw, err
+        out[name] = res
+        del A, B, C0, opA, opB, ref, bound
+        torch.cuda.empty_cache()
+
+    for tr in ("NT", "TN", "TT"):
+        run(f"sgemm_8192_{tr}", int(tr[0] == "T"), int(tr[1] == "T"), 8192, 8192, 8192, 8192, 8192, 8192, 0.0)
+    run("sgemm_4096x8192x2048_ld4160_NN", 0, 0, 4096, 8192, 2048, 4160, 4160, 4160, 0.0)
+    run("sgemm_256_NN_beta0.5", 0, 0, 256, 256, 256, 256, 256, 256, 0.5)
+    h.set_math_mode(ib.MATH_FP32)
+    return out
+
+
+def ctrsm_rows(h, ib, stream, steps: int) -> dict:
+    """CTRSM left/lower/{N, C}/non-unit 4096 x 1024 (complex), FP32 path;
+    forward error against a complex FP64 solve computed with torch on the GPU.
+    Useful flops 4 m^2 n (one complex multiply-add = 8 real flops, m^2 n / 2 of them)."""
+    import torch
+
+    from clgpu_amd import datagen
+    dev = torch.device("cuda:0")
+    h.set_math_mode(ib.MATH_FP32)
+    h.set_implementation(None)
+    m, nr = 4096, 1024
+    T = datagen.complex_triangular(m, lower=True, seed=0, poison=True)
+    Bm = datagen.complex_matrix(m, nr, name="B", seed=0)
+    tT = torch.from_numpy(np.ascontiguousarray(T.T)).to(dev)
+    tB = torch.from_numpy(np.ascontiguousarray(Bm.T)).to(dev)
+    L64 = torch.nan_to_num(tT.to(torch.complex128).t()).tril()
+    out = {}
+    for label, trans in (("NN", 0), ("CN", 2)):
+        opL = L64 if trans == 0 else L64.conj().t()
+        X64 = torch.linalg.solve_triangular(opL, tB.to(torch.complex128).t(), upper=trans != 0)
+        X = tB.clone()
+        ib.check(ib.iclblasCtrsm(h.value, 0, 1, trans, 0, m, nr, 1.0, tT.data_ptr(), m, X.data_ptr(), m))
+        torch.cuda.synchronize()
+        fe = float((X.to(torch.complex128).t() - X64).norm() / X64.norm())
+        Xw = tB.clone()
+        ms = _time(stream, lambda: ib.check(ib.iclblasCtrsm(h.value, 0, 1, trans, 0, m, nr, 1.0, tT.data_ptr(), m,
+                                                            Xw.data_ptr(), m)), max(2, steps // 4), 1)
+        tf = 4.0 * m * m * nr / (ms * 1e-3) / 1e12
+        out[label] = {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
+                      "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA, "forward_rel_err_fro": fe}
+        del X, Xw, X64, opL
+    del tT, tB, L64
+    torch.cuda.empty_cache()
+    return out
+
+
+def chain_rows(h, ib, stream, steps: int) -> dict:
+    """The differentiable chain X = A^-1 (alpha G H), loss = sum(W * X) with A
+    2048^2 lower triangular, G 2048^2, H 2048 x 512, alpha = 0.5: one step is the
+    forward pass (SGEMM, STRSM) and the backward pass (STRSM adjoint: dY and the
+    masked dA; SGEMM adjoint: dG, dH) through the C ABI on device pointers.
+    Useful flops: forward 2 n^2 r + n^2 r, backward n^2 r + 3 * 2 n^2 r."""
+    import torch
+
+    from clgpu_amd import datagen
+    dev = torch.device("cuda:0")
+    h.set_math_mode(ib.MATH_FP32)
+    h.set_implementation(None)
+    n, r, alpha = 2048, 512, 0.5
+
+    def t(x):  # column-major host matrix -> torch (cols, ld) on the GPU
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(x, np.float32).T)).to(dev)
+
+    A = t(datagen.triangular(n, lower=True, seed=0, poison=False))
+    G = t(datagen.matrix(n, n, name="G", seed=0))
+    H = t(datagen.matrix(n, r, name="H", seed=0))
+    W = t(datagen.normal(n * r, seed=0, stream=5).reshape(r, n).T)
+    X = torch.empty((r, n), device=dev)
+    dY = torch.empty((r, n), device=dev)
+    dA = torch.empty((n, n), device=dev)
+    dG = torch.empty((n, n), device=dev)
+    dH = torch.empty((r, n), device=dev)
+
+    def step():
+        ib.check(ib.iclblasSgemm(h.value, 0, 0, n, r, n, alpha, G.data_ptr(), n, H.data_ptr(), n, 0.0, X.data_ptr(),
+                                 n), "iclblasSgemm")
+        ib.check(ib.iclblasStrsm(h.value, 0, 1, 0, 0, n, r, 1.0, A.data_ptr(), n, X.data_ptr(), n), "iclblasStrsm")
+        ib.check(ib.iclblasStrsmBackward(h.value, 0, 1, 0, 0, n, r, 1.0, A.data_ptr(), n, X.data_ptr(), n,
+                                         W.data_ptr(), n, dA.data_ptr(), n, dY.data_ptr(), n), "iclblasStrsmBackward")
+        ib.check(ib.iclblasSgemmBackward(h.value, 0, 0, n, r, n, alpha, G.data_ptr(), n, H.data_ptr(), n,
+                                         dY.data_ptr(), n, dG.data_ptr(), n, dH.data_ptr(), n), "iclblasSgemmBackward")
+
+    step()
+    torch.cuda.synchronize()
+    # FP64 check of the loss and of dH (dH = alpha G^T A^-T W)
+    L64 = A.double().t().tril()
+    X64 = torch.linalg.solve_triangular(L64, alpha * (G.double().t() @ H.double().t()), upper=False)
+    loss64 = float((W.double().t() * X64).sum())
+    loss = float((W.double() * X.double()).sum())
+    dY64 = torch.linalg.solve_triangular(L64.t(), W.double().t(), upper=True)
+    dH64 = alpha * G.double() @ dY64  # (G^T)^T storage: G.t() is G, so G^T dY64 = G_storage @ dY64
+    dH_err = float((dH.double().t() - dH64).norm() / dH64.norm())
+    ms = _time(stream, step, max(2, steps // 2), 1)
+    flops = 3.0 * n * n * r + 7.0 * n * n * r
+    out = {"ms": ms, "tflops": flops / (ms * 1e-3) / 1e12, "loss_rel_err": abs(loss - loss64) / abs(loss64),
+           "dH_rel_err": dH_err}
+    del A, G, H, W, X, dY, dA, dG, dH, L64, X64, dY64, dH64
+    torch.cuda.empty_cache()
+    return out
 
 
 def cgemm_rows(h, ib, stream, steps: int) -> dict:
@@ -298,7 +444,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     rows = ["sgemm"] if a.no_extra else [r for r in a.rows.split(",") if r]
-    unknown = set(rows) - {"sgemm", "cgemm", "strsm"}
+    unknown = set(rows) - {"sgemm", "sgemm_more", "cgemm", "strsm", "ctrsm", "chain"}
     if unknown:
         print(f"bench.py: unknown rows {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
@@ -330,12 +476,21 @@ def main():
                 "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                 "dtype": "f32", "data": "synthetic", "config": {"workload": "secondary rows only: " + ",".join(rows)}}
     other = {}
+    if "sgemm_more" in rows:
+        print("[bench] other SGEMM rows", file=sys.stderr, flush=True)
+        other.update(sgemm_rows(h, ib, stream, a.steps))
     if "cgemm" in rows:
         print("[bench] CGEMM rows", file=sys.stderr, flush=True)
         other["cgemm_4096"] = cgemm_rows(h, ib, stream, a.steps)
     if "strsm" in rows:
         print("[bench] STRSM rows", file=sys.stderr, flush=True)
         other["strsm_4096x1024"] = strsm_rows(h, ib, stream, a.steps)
+    if "ctrsm" in rows:
+        print("[bench] CTRSM rows", file=sys.stderr, flush=True)
+        other["ctrsm_4096x1024"] = ctrsm_rows(h, ib, stream, a.steps)
+    if "chain" in rows:
+        print("[bench] GEMM+TRSM chain row", file=sys.stderr, flush=True)
+        other["chain_2048x512"] = chain_rows(h, ib, stream, a.steps)
     if other:
         line["other_rows"] = other
     if not a.no_cpu_baseline and A is not None:

@@ -441,9 +441,12 @@ __global__ void __launch_bounds__(256) trsm_step_kernel(icl_trsm_blocked_args a,
 // every wait ends (by induction from ticket 0) however many workgroups are
 // resident. Each wait is also bounded: past kDfSpin polls it sets the abort
 // word, and every later wait returns at once, so the launch always drains.
-// Producers publish a tile with release fences at agent scope before setting
-// its flag; consumers acquire at agent scope after seeing it (the tiles may be
-// produced on another XCD, whose L2 is not coherent with this one).
+// A wave polls up to 64 flags at a time and skips the polls (and acquires) of
+// the steps whose producers it has already seen finished.
+// A producer publishes a tile from one wave, with a release fence at agent
+// scope between the tile's stores and its flag; consumers acquire at agent
+// scope after seeing the flag (the tiles may be produced on another XCD, whose
+// L2 is not coherent with this one).
 // X is tile-major; a tile stores its 64 x 64 values with the dimension that is
 // contiguous in B fastest (lines when LINES_CONTIG, rows otherwise).
 // ---------------------------------------------------------------------------
@@ -456,22 +459,32 @@ __device__ __forceinline__ int df_poll(const int* p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Every lane of a wave calls this with the same flag. Returns false when the
-// solve was aborted (then nothing may be assumed about the tile).
-__device__ __forceinline__ bool df_wait(const int* flag, int* abort_word, int* error_word)
+// Every lane of a wave calls this with the same j and n. Waits until the flag of
+// tile j (flags[j * stride]) is set, polling the flags of tiles j .. j + 63 (< n)
+// at once, one per lane, and returns the end of the run of set flags starting at
+// j, so the steps up to it need no further poll. One acquire at agent scope
+// covers the whole run. Returns n when the solve was aborted (then nothing may be
+// assumed about the tiles).
+__device__ __forceinline__ int df_ready(const int* flags, long stride, int j, int n, int lane, int* abort_word,
+                                        int* error_word)
 {
+    const int jj = j + lane;
     for (int it = 0;; ++it) {
-        if (df_poll(flag) != 0) break;
-        if (df_poll(abort_word) != 0) return false;
+        const int f = jj < n ? df_poll(flags + (long)jj * stride) : 0;
+        const unsigned long long set = __ballot(f != 0);
+        const int run = ~set == 0ull ? 64 : __builtin_ctzll(~set);
+        if (run > 0) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return j + run;
+        }
+        if (df_poll(abort_word) != 0) return n;
         if (it >= kDfSpin) {
             __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(error_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
+            return n;
         }
         __builtin_amdgcn_s_sleep(2);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return true;
 }
 
 template <bool LINES_CONTIG>
@@ -531,32 +544,41 @@ __global__ void __launch_bounds__(256) trsm_dataflow_kernel(icl_trsm_blocked_arg
         }
     }
 
-    // sum_{j<b} L_bj X_jc, the next pair of tiles read into registers while the
-    // current one is multiplied
+    // sum_{j<b} L_bj X_jc. The (L, X) pair of step j + 1 is read into registers
+    // during the multiplication of step j when its producer is already known to
+    // have finished; otherwise step j is multiplied first and the wait for tile
+    // j + 1 comes after it, so that only the read and the multiplication of the
+    // newest tile lie between its publication and this tile's.
     const float* l_row = a.L + (long)r0 * a.l_si;
     const float* l_diag = l_row + (long)r0 * a.l_sk;
     TileRegs<kNB, kNB, L_ROWS_CONTIG> lt;
     TileRegs<kNB, kNB, LINES_CONTIG> xt;
     f32x16 acc = f32x16{};
-    if (b > 0) {
-        lt.load(l_row, a.l_si, a.l_sk, a.nsys - r0, kNB, tid, l_diag);
-        df_wait(&flags[c], abort_word, a.error_word);
-        xt.load(xtile(0), s_l, s_k, kNB, kNB, tid, a.X);
-    }
+    int ready = 0; // tiles (j, c) with j < ready are known to be published (uniform per wave)
+    auto fetch = [&](int j) {
+        lt.load(l_row + (long)j * kNB * a.l_sk, a.l_si, a.l_sk, a.nsys - r0, kNB, tid, l_diag);
+        if (j >= ready) ready = df_ready(flags + c, ncb, j, b, lane, abort_word, a.error_word);
+        xt.load(xtile(j), s_l, s_k, kNB, kNB, tid, a.X);
+    };
+    auto multiply = [&]() {
+#pragma unroll 8
+        for (int kk = 0; kk < kNB; kk += 2) {
+            const int kr = kk + (lane >> 5);
+            acc = df_mfma<LINES_CONTIG>(Ls[kr * kDfLd + wr + (lane & 31)], Xs[kr * kDfLd + wc + (lane & 31)], acc);
+        }
+    };
+    if (b > 0) fetch(0);
     for (int j = 0; j < b; ++j) {
         __syncthreads(); // every wave is done with the tiles of step j - 1
         lt.template store<kDfLd>(Ls, tid);
         xt.template store<kDfLd>(Xs, tid);
         __syncthreads();
-        if (j + 1 < b) {
-            lt.load(l_row + (long)(j + 1) * kNB * a.l_sk, a.l_si, a.l_sk, a.nsys - r0, kNB, tid, l_diag);
-            df_wait(&flags[(j + 1) * ncb + c], abort_word, a.error_word);
-            xt.load(xtile(j + 1), s_l, s_k, kNB, kNB, tid, a.X);
-        }
-#pragma unroll 8
-        for (int kk = 0; kk < kNB; kk += 2) {
-            const int kr = kk + (lane >> 5);
-            acc = df_mfma<LINES_CONTIG>(Ls[kr * kDfLd + wr + (lane & 31)], Xs[kr * kDfLd + wc + (lane & 31)], acc);
+        if (j + 1 < b && j + 1 < ready) { // producer done: overlap the read with this step
+            fetch(j + 1);
+            multiply();
+        } else {
+            multiply();
+            if (j + 1 < b) fetch(j + 1);
         }
     }
 
@@ -576,22 +598,43 @@ __global__ void __launch_bounds__(256) trsm_dataflow_kernel(icl_trsm_blocked_arg
         x = df_mfma<LINES_CONTIG>(Ds[kr * kLdD + wr + (lane & 31)], Xs[kr * kDfLd + wc + (lane & 31)], x);
     }
 
-    // publish X_bc (whole tile; rows / lines beyond the matrix are zero), then B = alpha X
-    float* xo = xtile(b);
+    // Publish X_bc with a single writer: the waves stage their quadrants in Ls
+    // as X[row][line]; wave 0 stores the whole tile (rows / lines beyond the
+    // matrix are zero), then its release fence at agent scope orders those stores
+    // before the flag. Then every wave writes B = alpha X.
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int d = 8 * (r >> 2) + (r & 3);
         const int row = row_t + (LINES_CONTIG ? d : 0), col = col_t + (LINES_CONTIG ? 0 : d);
-        xo[row * s_k + col * s_l] = x[r];
+        Ls[row * kDfLd + col] = x[r];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&flags[t], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (wave == 0) {
+        float4* xo = reinterpret_cast<float4*>(xtile(b));
+#pragma unroll
+        for (int i = 0; i < kNB * kNB / 4 / 64; ++i) {
+            const int q = lane + 64 * i, e = 4 * q;
+            // tile element e..e+3 along the fast dimension: (k, l..l+3) or (k..k+3, l)
+            const int fast = e % kNB, slow = e / kNB;
+            float4 v;
+            if (LINES_CONTIG) { // e = k * 64 + l
+                v = make_float4(Ls[slow * kDfLd + fast], Ls[slow * kDfLd + fast + 1], Ls[slow * kDfLd + fast + 2],
+                                Ls[slow * kDfLd + fast + 3]);
+            } else { // e = l * 64 + k
+                v = make_float4(Ls[fast * kDfLd + slow], Ls[(fast + 1) * kDfLd + slow], Ls[(fast + 2) * kDfLd + slow],
+                                Ls[(fast + 3) * kDfLd + slow]);
+            }
+            xo[q] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0) __hip_atomic_store(&flags[t], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int d = 8 * (r >> 2) + (r & 3);
         const int dr = LINES_CONTIG ? d : 0, dc = LINES_CONTIG ? 0 : d;
-        if (r0 + row_t + dr < a.nsys && col_t + dc < ncols) pb[dr * a.x_si + dc * a.x_sj] = alpha * x[r];
+        if (r0 + row_t + dr < a.nsys && col_t + dc < ncols)
+            __builtin_nontemporal_store(alpha * x[r], pb + dr * a.x_si + dc * a.x_sj);
     }
 }
 
