@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -41,11 +42,13 @@ bool gemm_needs_edge(const icl_gemm_variant& v, const GemmProblem& p)
 // Score = 1 / estimated run time (seconds). Each variant carries its measured
 // full-GPU rate R (TFLOP/s at 8192^3) and occupancy occ (workgroups per CU).
 // With 256 CUs, 256*occ tiles run concurrently, each tile taking
-// t_tile = 2*bm*bn*K*flops_per_mac_scale / (R / (256*occ)); the grid runs in
-// ceil(tiles / (256*occ)) such rounds, so small or ragged problems pay for idle
-// CUs and padded tiles. Variants with a pack prepass add its HBM time
-// ((M + N) * K * (4 read + 4 written) bytes at 5 TB/s). The bounds-checked
-// instantiation is taken to run 10% slower.
+// t_tile = 2*bm*bn*K*flops_per_mac_scale / (R / (256*occ)) when the CUs are
+// full; the grid runs in ceil(tiles / (256*occ)) rounds, so ragged problems pay
+// for padded tiles. A last round with fewer than 256*occ tiles shares the CUs
+// among fewer workgroups: it is taken to last t_tile * max(tiles_last, 256) /
+// (256*occ), so grids of fewer tiles than CUs favour smaller tiles. Variants
+// with a pack prepass add its HBM time ((M + N) * K * (4 read + 4 written)
+// bytes at 5 TB/s). The bounds-checked instantiation is taken to run 10% slower.
 float gemm_score(const icl_gemm_variant& v, const GemmProblem& p)
 {
     const icl_gemm_args& g = *p.args;
@@ -55,7 +58,9 @@ float gemm_score(const icl_gemm_variant& v, const GemmProblem& p)
     const double tiles = double((g.M + v.bm - 1) / v.bm) * double((g.N + v.bn - 1) / v.bn);
     const double rate = (v.tflops_hint > 0 ? v.tflops_hint : 1.0) * 1e12;
     const double t_tile = 2.0 * macs * v.bm * v.bn * double(g.K) * conc / rate;
-    double t = std::ceil(tiles / conc) * t_tile;
+    const double rounds = std::ceil(tiles / conc);
+    const double last = tiles - (rounds - 1.0) * conc;
+    double t = (rounds - 1.0) * t_tile + t_tile * std::max(last, kCUs) / conc;
     if (gemm_needs_edge(v, p)) t *= 1.10;
     if (v.workspace) t += (double(g.M) + double(g.N)) * double(g.K) * 8.0 * macs / 5e12;
     return (float)(1.0 / t);
@@ -174,10 +179,12 @@ void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int 
 namespace {
 
 constexpr int kLeaf = 64;
-// Whether the single-launch dataflow solve, rather than the launch-per-step
-// blocked solve, runs when no implementation is forced (both are selectable
-// by name).
-constexpr bool kTrsmDataflowDefault = false;
+// Blocked STRSM above 64 unknowns: the panel solve (256 unknowns per launch
+// plus a GEMM update of the rest) runs when no implementation is forced and
+// kTrsmPanelDefault is set; the launch-per-64-block solve and the single-launch
+// dataflow solve are selectable by name.
+constexpr bool kTrsmPanelDefault = true;
+constexpr int kPanel = 256;
 
 struct Trsm {
     iclblasContext* h;
@@ -466,9 +473,49 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
         }
         const int nblk = (order + kLeaf - 1) / kLeaf;
         const size_t inv_elems = (size_t)nblk * kLeaf * kLeaf;
-        const bool dataflow = h->forced_impl == "strsm_dataflow64" ||
-                              (kTrsmDataflowDefault && h->forced_impl != "strsm_blocked_inv64");
-        if (dataflow) {
+        const bool dataflow = h->forced_impl == "strsm_dataflow64";
+        const bool panel = h->forced_impl == "strsm_panel256" ||
+                           (kTrsmPanelDefault && h->forced_impl != "strsm_blocked_inv64" && !dataflow);
+        if (panel) {
+            // X in original order with B's shape (m x n, ld m); the canonical X view
+            // mirrors B's. Panels of 256 unknowns are solved in one launch each, and
+            // one FP32 GEMM per panel updates every later unknown, in original
+            // coordinates (positive strides).
+            float* ws = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
+                (inv_elems + (size_t)m * n) * sizeof(float), h->stream));
+            float* Xb = ws + inv_elems;
+            a.invD = ws;
+            if (t.left) {
+                a.X = t.fwd ? Xb : Xb + e;
+                a.xv_si = t.fwd ? 1 : -1;
+                a.xv_sj = m;
+            } else {
+                a.X = t.fwd ? Xb : Xb + e * m;
+                a.xv_si = t.fwd ? (long)m : -(long)m;
+                a.xv_sj = 1;
+            }
+            hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
+            for (int ps = 0; ps < order; ps += kPanel) {
+                const int pe = std::min(order, ps + kPanel);
+                hip_check(icl_trsm_panel(&a, ps, pe, h->stream), "trsm_panel");
+                if (pe == order) break;
+                const int plen = pe - ps, rlen = order - pe;
+                const int p_lo = t.fwd ? ps : order - pe; // original index of the panel's first unknown
+                const int r_lo = t.fwd ? pe : 0;          // original index of the first later unknown
+                int op = 0;
+                if (t.left) {
+                    const float* As = t.sub(r_lo, p_lo, &op);
+                    sgemm_dev(h, ICL_MATH_FP32, op, 0, rlen, n, plen, -1.0f, As, lda, Xb + p_lo, m, 1.0f, B + r_lo, ldb,
+                              ICL_MASK_NONE, false);
+                } else {
+                    const float* As = t.sub(p_lo, r_lo, &op);
+                    sgemm_dev(h, ICL_MATH_FP32, 0, op, m, rlen, plen, -1.0f, Xb + (long)p_lo * m, m, As, lda, 1.0f,
+                              B + (long)r_lo * ldb, ldb, ICL_MASK_NONE, false);
+                }
+            }
+            hip_check(icl_axpby_2d(m, n, alpha, Xb, m, 0.0f, B, ldb, h->stream), "axpby");
+            if (record) h->last_impl = "strsm_panel256";
+        } else if (dataflow) {
             // invD, the tile-major X (whole 64 x 64 tiles), then the flags / ticket / abort words
             const int ncb = (a.nlines + kLeaf - 1) / kLeaf;
             const size_t x_elems = (size_t)nblk * ncb * kLeaf * kLeaf;
@@ -477,7 +524,7 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
                 (inv_elems + x_elems) * sizeof(float) + sync_ints * sizeof(int), h->stream));
             a.invD = reinterpret_cast<float*>(ws);
             a.X = a.invD + inv_elems;
-            a.ldx = 0;
+            a.xv_si = a.xv_sj = 0;
             a.sync = reinterpret_cast<int*>(a.X + x_elems);
             if (!h->trsm_error) {
                 void* w = nullptr;
@@ -494,7 +541,8 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
                 (inv_elems + (size_t)order * a.nlines) * sizeof(float), h->stream));
             a.invD = ws;
             a.X = ws + inv_elems;
-            a.ldx = order;
+            a.xv_si = 1;
+            a.xv_sj = order;
             hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
             for (int b = 0; b < nblk; ++b) hip_check(icl_trsm_step(&a, b, h->stream), "trsm_step");
             hip_check(icl_trsm_scatter(&a, alpha, h->stream), "trsm_scatter");

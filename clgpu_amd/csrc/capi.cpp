@@ -28,10 +28,12 @@ namespace {
 
 enum Slot { kSlotA = 0, kSlotB = 1, kSlotC = 2, kSlotW0 = 3, kSlotW1 = 4, kSlotW2 = 5 };
 
-// STRSM implementations (selected by size; forceable by name): the blocked solve
-// with inverted 64 x 64 diagonal blocks (default above 64 unknowns) and the
-// recursive split into diagonal-block solves plus GEMM updates.
-const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_dataflow64", "strsm_recursive"};
+// STRSM implementations above 64 unknowns (forceable by name): the panel solve
+// (default: 256 unknowns per launch, the rest updated by a GEMM), the
+// launch-per-64-block solve with inverted diagonal blocks, the single-launch
+// dataflow solve, and the recursive split into diagonal-block solves plus GEMM
+// updates.
+const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_dataflow64", "strsm_panel256", "strsm_recursive"};
 
 template <typename F>
 iclblasStatus_t guarded(F&& f)

@@ -99,9 +99,10 @@ hipError_t icl_ctrsm_diag(const icl_ctrsm_diag_args* a, hipStream_t stream);
  * unknowns and nlines right-hand sides: L(i, k) = L[i * l_si + k * l_sk],
  * B(i, j) = B[i * x_si + j * x_sj] (strides may be negative). The 64 x 64
  * diagonal blocks are inverted into invD (ceil(nsys/64) * 4096 floats); step b
- * forms X_b = inv(L_bb) B_b into the dense X (ld = ldx) and subtracts L_rb X_b
- * from the trailing rows of B (B is used as scratch); scatter writes
- * alpha * X back through the view of B. */
+ * forms X_b = inv(L_bb) B_b into X (X(i, j) = X[i * xv_si + j * xv_sj]) and
+ * subtracts L_rb X_b from the trailing rows of B (B is used as scratch);
+ * scatter writes alpha * X back through the view of B. The panel solve works on
+ * rows [ps, pe) of the same views (see icl_trsm_panel). */
 typedef struct icl_trsm_blocked_args {
     int nsys, nlines;
     const float* L;
@@ -109,7 +110,7 @@ typedef struct icl_trsm_blocked_args {
     float* B;
     long x_si, x_sj;
     float* X;
-    long ldx;
+    long xv_si, xv_sj;
     float* invD;
     int unit;
     int* sync;       /* dataflow solve only: icl_trsm_dataflow_sync_ints(...) ints, zeroed by invert_blocks */
@@ -121,8 +122,8 @@ hipError_t icl_trsm_scatter(const icl_trsm_blocked_args* a, float alpha, hipStre
 
 /* The whole blocked solve in one launch. X is tile-major here: the 64 x 64
  * tile (row block j, line block c) holds X(64 j + k, 64 c + l) at
- * X[(j * ncb + c) * 4096 + k * 64 + l], ncb = ceil(nlines / 64); ldx is
- * unused. One workgroup per tile takes tiles in row-block order from a ticket
+ * X[(j * ncb + c) * 4096 + k * 64 + l], ncb = ceil(nlines / 64); xv_si and
+ * xv_sj are unused. One workgroup per tile takes tiles in row-block order from a ticket
  * counter, accumulates L_bj X_jc over j < b as the producers' flags are set,
  * then X_bc = inv(L_bb) (B_bc - sum), stores it, sets its flag and writes
  * alpha * X_bc through the view of B. invert_blocks (same stream, just before)
@@ -131,6 +132,14 @@ hipError_t icl_trsm_scatter(const icl_trsm_blocked_args* a, float alpha, hipStre
  * which every wait returns at once, so the launch always drains; the result is
  * then invalid, and *error_word is set for the host to report. */
 int icl_trsm_dataflow_sync_ints(int nsys, int nlines);
+
+/* Solve of the diagonal panel rows [ps, pe) (pe - ps <= 256, ps a multiple of
+ * 64) of the canonical system, for all lines: one workgroup per 64 lines runs
+ * the panel's 64-row blocks in order inside the workgroup (X_i = inv(L_ii) B_i,
+ * then B_k -= L_ki X_i for the later blocks k of the panel) and writes X rows
+ * [ps, pe) through the X view. B rows of the panel are read, not written; rows
+ * outside the panel are not touched (the caller updates them, e.g. with a GEMM). */
+hipError_t icl_trsm_panel(const icl_trsm_blocked_args* a, int ps, int pe, hipStream_t stream);
 hipError_t icl_trsm_dataflow(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream);
 
 /* R = fp32(alpha * B0 - P * Q) with the products and the sum in FP64.

@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(256) trsm_step_kernel(icl_trsm_blocked_args a,
         for (int r = 0; r < 16; ++r) {
             const int col = wc + icl::mfma32_row(lane, r);
             BX[row * kLdB + col] = x[r];
-            if (g == 0 && r0 + row < a.nsys && col < ncols) a.X[(long)(r0 + row) + (long)(c0 + col) * a.ldx] = x[r];
+            if (g == 0 && r0 + row < a.nsys && col < ncols) a.X[(long)(r0 + row) * a.xv_si + (long)(c0 + col) * a.xv_sj] = x[r];
         }
     }
     if (!update) return;
@@ -638,6 +638,124 @@ __global__ void __launch_bounds__(256) trsm_dataflow_kernel(icl_trsm_blocked_arg
     }
 }
 
+// ---------------------------------------------------------------------------
+// Panel solve: rows [ps, pe) of the canonical system (at most four 64-row
+// blocks), one workgroup per 64 lines, entirely inside the workgroup: the
+// panel of B is staged in LDS as P[row][line]; block i is solved as
+// X_i = inv(L_ii) B_i and subtracted from the later blocks of the panel
+// (B_k -= L_ki X_i), each a 64 x 64 x 64 product on v_mfma_f32_32x32x2_f32
+// (one 32 x 32 quadrant per wave).
+// ---------------------------------------------------------------------------
+constexpr int kPanelRows = 4 * kNB;
+constexpr int kPanelLd = kStepCols + 1;
+constexpr size_t kPanelLds = sizeof(float) * ((size_t)kPanelRows * kPanelLd + (size_t)kNB * kDfLd);
+
+template <bool LINES_CONTIG, bool L_ROWS_CONTIG>
+__global__ void __launch_bounds__(256) trsm_panel_kernel(icl_trsm_blocked_args a, int ps, int pe)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    float* P = sm;                      // P[row][line]: panel rows of B, then of X
+    float* T = P + kPanelRows * kPanelLd; // T[k][r]: operand tile of the current product
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c0 = blockIdx.x * kStepCols;
+    const int ncols = min(kStepCols, a.nlines - c0);
+    const int np = pe - ps;
+    const int nsub = (np + kNB - 1) / kNB;
+    const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1); // quadrant of this wave: rows wr.., lines wc..
+
+    // the panel of B: element (line l, row k) at B[(ps + k) * x_si + (c0 + l) * x_sj]
+    const float* bsrc = a.B + (long)ps * a.x_si + (long)c0 * a.x_sj;
+    TileRegs<kNB, kStepCols, LINES_CONTIG> bt[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < nsub) bt[i].load(bsrc + (long)(kNB * i) * a.x_si, a.x_sj, a.x_si, ncols, np - kNB * i, tid, bsrc);
+
+    // The operand tiles of the (at most ten) products, in product order
+    // inv(L_00), L_10, L_20, L_30, inv(L_11), L_21, L_31, inv(L_22), L_32,
+    // inv(L_33), are all read into registers up front, so the products wait for
+    // one memory latency in total rather than one each. Product q works on
+    // block kPi[q] of X and block kPk[q] of B (the inverse when they are equal).
+    // L_ki is read through the L view, inv(L_ii) from its dense column-major
+    // block (element (r, k) at invD[k * 64 + r]); either way the store gives
+    // T[k][r] = operand(r, k).
+    constexpr int kPi[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+    constexpr int kPk[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+    TileRegs<kNB, kNB, L_ROWS_CONTIG> ot[10];
+    const float* l_base = a.L + (long)ps * (a.l_si + a.l_sk);
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+        const int i = kPi[q], k = kPk[q];
+        if (k < nsub) {
+            if (i == k)
+                ot[q].load(a.invD + (long)(ps / kNB + i) * kNB * kNB, 1, kNB, kNB, kNB, tid, a.invD);
+            else
+                ot[q].load(l_base + (long)(kNB * k) * a.l_si + (long)(kNB * i) * a.l_sk, a.l_si, a.l_sk,
+                           np - kNB * k, kNB, tid, l_base);
+        }
+    }
+    // with every read in flight, stage the panel of B
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < nsub) bt[i].template store<kPanelLd>(P + kNB * i * kPanelLd, tid);
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+        const int i = kPi[q], k = kPk[q];
+        if (k >= nsub) continue; // uniform: the panel has fewer blocks
+        __syncthreads();         // the previous product is done with T (and P is staged)
+        ot[q].template store<kDfLd>(T, tid);
+        __syncthreads();
+        float* Xi = P + kNB * i * kPanelLd; // rows of block i
+        if (k == i) {
+            // X_i = inv(L_ii) B_i: lanes along rows of the result
+            f32x16 x = f32x16{};
+#pragma unroll 8
+            for (int kk = 0; kk < kNB; kk += 2) {
+                const int kr = kk + (lane >> 5);
+                x = icl::mfma_32x32x2_f32(Xi[kr * kPanelLd + wc + (lane & 31)], T[kr * kDfLd + wr + (lane & 31)], x);
+            }
+            __syncthreads(); // every wave has read B_i
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                Xi[(wr + (lane & 31)) * kPanelLd + wc + icl::mfma32_row(lane, r)] = x[r];
+        } else {
+            // B_k -= L_ki X_i
+            float* Bk = P + kNB * k * kPanelLd;
+            f32x16 u;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) u[r] = Bk[(wr + (lane & 31)) * kPanelLd + wc + icl::mfma32_row(lane, r)];
+#pragma unroll 8
+            for (int kk = 0; kk < kNB; kk += 2) {
+                const int kr = kk + (lane >> 5);
+                u = icl::mfma_32x32x2_f32(Xi[kr * kPanelLd + wc + (lane & 31)], -T[kr * kDfLd + wr + (lane & 31)], u);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) Bk[(wr + (lane & 31)) * kPanelLd + wc + icl::mfma32_row(lane, r)] = u[r];
+        }
+    }
+    __syncthreads();
+
+    // X rows [ps, pe) through the X view, lanes along its contiguous dimension:
+    // 64 elements per thread, addresses from one base plus uniform strides.
+    {
+        float* dst = a.X + (long)ps * a.xv_si + (long)c0 * a.xv_sj;
+        if (LINES_CONTIG) { // thread: line tid % 64, rows tid / 64 + 4 j
+            const int l = tid % kStepCols, r0t = tid / kStepCols;
+            float* d = dst + (long)r0t * a.xv_si + (long)l * a.xv_sj;
+#pragma unroll
+            for (int j = 0; j < kPanelRows / 4; ++j) {
+                const int r = r0t + 4 * j;
+                if (r < np && l < ncols) d[(long)(4 * j) * a.xv_si] = P[r * kPanelLd + l];
+            }
+        } else { // thread: row tid, all lines
+            const int r = tid;
+            float* d = dst + (long)r * a.xv_si;
+#pragma unroll
+            for (int l = 0; l < kStepCols; ++l)
+                if (r < np && l < ncols) d[(long)l * a.xv_sj] = P[r * kPanelLd + l];
+        }
+    }
+}
+
 // B(i, j) = alpha * X(i, j) through the canonical view of B.
 __global__ void trsm_scatter_kernel(icl_trsm_blocked_args a, float alpha)
 {
@@ -652,7 +770,7 @@ __global__ void trsm_scatter_kernel(icl_trsm_blocked_args a, float alpha)
             i = e / a.nlines;
             j = e - i * a.nlines;
         }
-        a.B[i * a.x_si + j * a.x_sj] = alpha * a.X[i + j * a.ldx];
+        a.B[i * a.x_si + j * a.x_sj] = alpha * a.X[i * a.xv_si + j * a.xv_sj];
     }
 }
 
@@ -764,6 +882,30 @@ extern "C" hipError_t icl_trsm_dataflow(const icl_trsm_blocked_args* a, float al
     const bool l_rows_contig = a->l_si == 1 || a->l_si == -1;
     hipLaunchKernelGGL(kernels[2 * lines_contig + l_rows_contig], dim3((unsigned)tiles), dim3(256), kDfLds, stream, *a,
                        alpha);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t icl_trsm_panel(const icl_trsm_blocked_args* a, int ps, int pe, hipStream_t stream)
+{
+    if (a->nlines <= 0 || pe <= ps) return hipSuccess;
+    if (ps % kNB != 0 || pe - ps > kPanelRows || pe > a->nsys) return hipErrorInvalidValue;
+    using Kern = void (*)(icl_trsm_blocked_args, int, int);
+    static const Kern kernels[4] = {trsm_panel_kernel<false, false>, trsm_panel_kernel<false, true>,
+                                    trsm_panel_kernel<true, false>, trsm_panel_kernel<true, true>};
+    static const hipError_t attr = [] {
+        for (Kern k : kernels) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPanelLds);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }();
+    if (attr != hipSuccess) return attr;
+    const bool lines_contig = a->x_sj == 1 || a->x_sj == -1;
+    const bool l_rows_contig = a->l_si == 1 || a->l_si == -1;
+    const unsigned blocks = (unsigned)((a->nlines + kStepCols - 1) / kStepCols);
+    hipLaunchKernelGGL(kernels[2 * lines_contig + l_rows_contig], dim3(blocks), dim3(256), kPanelLds, stream, *a, ps,
+                       pe);
     return hipGetLastError();
 }
 
