@@ -291,18 +291,22 @@ struct Trsm {
     }
 };
 
-// CTRSM: the same recursive blocking on interleaved complex data; strides and
-// offsets count complex elements. The coupling blocks go through the FP32 CGEMM
-// with op N, T or C as in op(A); leaves are complex diagonal-block solves.
+// CTRSM: recursive blocking on interleaved complex data; strides and offsets
+// count complex elements. The coupling blocks go through the FP32 CGEMM with op
+// N, T or C as in op(A). The splits keep every leaf on a 64-unknown block of the
+// canonical (forward) order -- backward solves split from the end -- so the
+// leaves are products with the diagonal blocks' inverses, all inverted up front
+// in one launch, rather than substitutions.
 struct CTrsm {
     iclblasContext* h;
     bool left, trans, conj, unit, fwd;
-    int m, n;
+    int m, n, order;
     const float* A;
     long lda;
     float* B;
     long ldb;
     long si, sk; // op(A)(p, q) = A[p * si + q * sk] (conjugated for OP_C)
+    const float* invD;
 
     const float* sub(int i0, int j0, int* op) const
     {
@@ -310,32 +314,37 @@ struct CTrsm {
         return A + 2 * ((long)i0 * si + (long)j0 * sk);
     }
 
+    // The canonical forward view of the whole system (unknown 0 solved first).
+    void canonical_l(const float** L, long* l_si, long* l_sk) const
+    {
+        const long e = fwd ? 0 : order - 1;
+        *L = A + 2 * e * (si + sk);
+        const long a_si = left ? si : sk, a_sk = left ? sk : si;
+        *l_si = fwd ? a_si : -a_si;
+        *l_sk = fwd ? a_sk : -a_sk;
+    }
+
     void leaf(int p0, int nb, const float s_post[2]) const
     {
-        icl_ctrsm_diag_args a{};
+        icl_ctrsm_apply_args a{};
         a.nb = nb;
-        a.unit = unit ? 1 : 0;
-        a.conj = conj ? 1 : 0;
+        const int blk = fwd ? p0 / kLeaf : (order - (p0 + nb)) / kLeaf; // canonical block of the leaf
+        a.invD = invD + (long)blk * kLeaf * kLeaf * 2;
         a.s_post_re = s_post[0];
         a.s_post_im = s_post[1];
         const long e = fwd ? p0 : p0 + nb - 1; // first unknown of the canonical order
-        a.L = A + 2 * e * (si + sk);
         if (left) {
             a.nlines = n;
             a.x_sj = ldb;
-            a.l_si = fwd ? si : -si;
-            a.l_sk = fwd ? sk : -sk;
             a.X = B + 2 * e;
             a.x_si = fwd ? 1 : -1;
         } else {
             a.nlines = m;
             a.x_sj = 1;
-            a.l_si = fwd ? sk : -sk;
-            a.l_sk = fwd ? si : -si;
             a.X = B + 2 * e * ldb;
             a.x_si = fwd ? ldb : -ldb;
         }
-        hip_check(icl_ctrsm_diag(&a, h->stream), "ctrsm_diag");
+        hip_check(icl_ctrsm_apply(&a, h->stream), "ctrsm_apply");
     }
 
     // Solve unknowns [p0, p1) (rows of X for left, columns for right).
@@ -347,26 +356,22 @@ struct CTrsm {
             return;
         }
         const int half = ((len / 2 + kLeaf - 1) / kLeaf) * kLeaf;
-        const int mid = p0 + half;
+        const int mid = fwd ? p0 + half : p1 - half; // the part solved first is a whole number of blocks
         const float minus_one[2] = {-1.0f, 0.0f}, one[2] = {1.0f, 0.0f};
+        const int first0 = fwd ? p0 : mid, first1 = fwd ? mid : p1;
+        const int next0 = fwd ? mid : p0, next1 = fwd ? p1 : mid;
         int op = 0;
+        solve(first0, first1, s_post);
         if (left) {
-            const int first0 = fwd ? p0 : mid, first1 = fwd ? mid : p1; // solved first
-            const int next0 = fwd ? mid : p0, next1 = fwd ? p1 : mid;
-            solve(first0, first1, s_post);
             const float* Asub = sub(next0, first0, &op);
             cgemm_dev(h, ICL_MATH_FP32, op, 0, next1 - next0, n, first1 - first0, minus_one, Asub, lda,
                       B + 2 * (long)first0, ldb, one, B + 2 * (long)next0, ldb);
-            solve(next0, next1, s_post);
         } else {
-            const int first0 = fwd ? p0 : mid, first1 = fwd ? mid : p1;
-            const int next0 = fwd ? mid : p0, next1 = fwd ? p1 : mid;
-            solve(first0, first1, s_post);
             const float* Asub = sub(first0, next0, &op);
             cgemm_dev(h, ICL_MATH_FP32, 0, op, m, next1 - next0, first1 - first0, minus_one,
                       B + 2 * (long)first0 * ldb, ldb, Asub, lda, one, B + 2 * (long)next0 * ldb, ldb);
-            solve(next0, next1, s_post);
         }
+        solve(next0, next1, s_post);
     }
 };
 
@@ -377,7 +382,7 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
 {
     if (m <= 0 || n <= 0) return;
     if (alpha[0] == 0.0f && alpha[1] == 0.0f) {
-        // X = 0 without reading A or B (the documented contract, reference iclBLAS.h:1713 for STRSM)
+        // X = 0 without reading A or B (the documented contract, reference iclBLAS.h:1713, :1924)
         hip_check(icl_caxpby_2d(m, n, 0.0f, 0.0f, nullptr, 0, 0.0f, 0.0f, B, ldb, h->stream), "caxpby");
         h->last_impl = "ctrsm_zero";
         return;
@@ -398,18 +403,52 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
     t.sk = t.trans ? 1 : lda;
     const bool lower_eff = (uplo == ICLBLAS_FILL_MODE_LOWER) != t.trans; // op(A) lower triangular
     t.fwd = t.left ? lower_eff : !lower_eff;
-    const int order = t.left ? m : n;
-    if (order <= kLeaf) {
-        // one diagonal block: alpha applied to the solution, as the reference kernel does
-        t.solve(0, order, alpha);
+    t.order = t.left ? m : n;
+    if (t.order <= kLeaf) {
+        // one diagonal block: the reference's substitution in registers, alpha
+        // applied to the solution as the reference kernel does (Ctrsm_naive.cl:35)
+        icl_ctrsm_diag_args a{};
+        a.nb = t.order;
+        a.unit = t.unit ? 1 : 0;
+        a.conj = t.conj ? 1 : 0;
+        a.s_post_re = alpha[0];
+        a.s_post_im = alpha[1];
+        long l_si = 0, l_sk = 0;
+        t.canonical_l(&a.L, &l_si, &l_sk);
+        a.l_si = l_si;
+        a.l_sk = l_sk;
+        const long e = t.fwd ? 0 : t.order - 1;
+        if (t.left) {
+            a.nlines = n;
+            a.x_sj = ldb;
+            a.X = B + 2 * e;
+            a.x_si = t.fwd ? 1 : -1;
+        } else {
+            a.nlines = m;
+            a.x_sj = 1;
+            a.X = B + 2 * e * ldb;
+            a.x_si = t.fwd ? ldb : -ldb;
+        }
+        hip_check(icl_ctrsm_diag(&a, h->stream), "ctrsm_diag");
         h->last_impl = "ctrsm_diag64";
         return;
     }
+    const int nblk = (t.order + kLeaf - 1) / kLeaf;
+    float* invD = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
+        (size_t)nblk * kLeaf * kLeaf * 2 * sizeof(float), h->stream));
+    icl_ctrsm_blocked_args ia{};
+    ia.nsys = t.order;
+    t.canonical_l(&ia.L, &ia.l_si, &ia.l_sk);
+    ia.unit = t.unit ? 1 : 0;
+    ia.conj = t.conj ? 1 : 0;
+    ia.invD = invD;
+    hip_check(icl_ctrsm_invert_blocks(&ia, h->stream), "ctrsm_invert_blocks");
+    t.invD = invD;
     const float one[2] = {1.0f, 0.0f};
-    t.solve(0, order, one);
+    t.solve(0, t.order, one);
     if (alpha[0] != 1.0f || alpha[1] != 0.0f)
         hip_check(icl_caxpby_2d(m, n, alpha[0], alpha[1], B, ldb, 0.0f, 0.0f, B, ldb, h->stream), "caxpby");
-    h->last_impl = "ctrsm_recursive";
+    h->last_impl = "ctrsm_recursive_inv64";
 }
 
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,

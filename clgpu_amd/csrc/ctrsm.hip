@@ -94,3 +94,152 @@ extern "C" hipError_t icl_ctrsm_diag(const icl_ctrsm_diag_args* a, hipStream_t s
     hipLaunchKernelGGL(ctrsm_diag_kernel, dim3(blocks), dim3(64), 0, stream, *a);
     return hipGetLastError();
 }
+
+namespace {
+
+// Inverts every 64 x 64 diagonal block of the canonical system (workgroup b
+// handles unknowns [64 b, 64 b + nb)): lane j solves L_bb y = e_j with the
+// reference's complex arithmetic and column j of inv(L_bb) is written to
+// invD[b * 4096 + j * 64 + i] (interleaved complex, rows and columns beyond
+// nb zero). L(i, k) = L[i * l_si + k * l_sk], conjugated when conj.
+__global__ void __launch_bounds__(64) ctrsm_invert_kernel(icl_ctrsm_blocked_args a)
+{
+    __shared__ CDiagBlock s;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    const int r0 = b * kNB;
+    const int nb = min(kNB, a.nsys - r0);
+    const float2* L = reinterpret_cast<const float2*>(a.L) + (long)r0 * (a.l_si + a.l_sk);
+    const float sg = a.conj ? -1.0f : 1.0f;
+    for (int i = 0; i < kNB; ++i) {
+        float2 v = make_float2(0.0f, 0.0f);
+        if (i < nb && lane < nb && lane > i) {
+            v = L[(long)lane * a.l_si + (long)i * a.l_sk];
+            v.y *= sg;
+        }
+        s.Lt[i][lane] = v;
+    }
+    if (lane < nb) {
+        float2 d = make_float2(1.0f, 0.0f);
+        if (!a.unit) {
+            d = L[(long)lane * (a.l_si + a.l_sk)];
+            d.y *= sg;
+        }
+        s.diag[lane] = d;
+    }
+    __syncthreads();
+    float2 x[kNB];
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) x[i] = make_float2(i == lane && lane < nb ? 1.0f : 0.0f, 0.0f);
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) {
+        if (i < nb) {
+            float2 xi = x[i];
+            if (!a.unit) xi = c_div(xi, s.diag[i]);
+            x[i] = xi;
+#pragma unroll
+            for (int k = i + 1; k < kNB; ++k) {
+                const float2 t = c_mul(s.Lt[i][k], xi);
+                x[k].x -= t.x;
+                x[k].y -= t.y;
+            }
+        }
+    }
+    float2* col = reinterpret_cast<float2*>(a.invD) + (long)b * kNB * kNB + (long)lane * kNB;
+#pragma unroll
+    for (int i = 0; i < kNB; ++i) col[i] = (lane < nb && i < nb) ? x[i] : make_float2(0.0f, 0.0f);
+}
+
+// X_b = s_post * inv(L_bb) B_b for one diagonal block and all lines, in place:
+// B(i, j) = X[i * x_si + j * x_sj] (interleaved complex, i < nb). One workgroup
+// per 64 lines; each wave forms a 32 x 32 quadrant of the product with four
+// real v_mfma_f32_32x32x2_f32 chains (Yr = Dr Br - Di Bi, Yi = Dr Bi + Di Br).
+// LINES_CONTIG: the lines are the contiguous dimension of B (right-side
+// solves), so the result's lanes run along lines for coalesced stores.
+constexpr int kApLd = kNB + 1;
+constexpr size_t kApLds = sizeof(float) * 4 * kNB * kApLd;
+template <bool LINES_CONTIG>
+__global__ void __launch_bounds__(256) ctrsm_apply_kernel(icl_ctrsm_apply_args a)
+{
+    extern __shared__ float sm[];
+    float(*Dr)[kApLd] = reinterpret_cast<float(*)[kApLd]>(sm); // D[k][r] = inv(L_bb)(r, k)
+    float(*Di)[kApLd] = Dr + kNB;
+    float(*Br)[kApLd] = Di + kNB; // B[k][l] = B_b(k, l)
+    float(*Bi)[kApLd] = Br + kNB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c0 = blockIdx.x * kNB;
+    const int ncols = min(kNB, a.nlines - c0);
+    const float2* inv = reinterpret_cast<const float2*>(a.invD);
+    float2* X = reinterpret_cast<float2*>(a.X);
+    for (int e = tid; e < kNB * kNB; e += 256) { // inv is dense column-major: e = k * 64 + r
+        const float2 v = inv[e];
+        Dr[e / kNB][e % kNB] = v.x;
+        Di[e / kNB][e % kNB] = v.y;
+    }
+    for (int e = tid; e < kNB * kNB; e += 256) {
+        const int k = LINES_CONTIG ? e / kNB : e % kNB, l = LINES_CONTIG ? e % kNB : e / kNB;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (k < a.nb && l < ncols) v = X[(long)k * a.x_si + (long)(c0 + l) * a.x_sj];
+        Br[k][l] = v.x;
+        Bi[k][l] = v.y;
+    }
+    __syncthreads();
+    const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
+    f32x16 yr = f32x16{}, yi = f32x16{};
+#pragma unroll 4
+    for (int kk = 0; kk < kNB; kk += 2) {
+        const int kr = kk + (lane >> 5);
+        const float dr = Dr[kr][wr + (lane & 31)], di = Di[kr][wr + (lane & 31)];
+        const float br = Br[kr][wc + (lane & 31)], bi = Bi[kr][wc + (lane & 31)];
+        if (LINES_CONTIG) { // lanes along lines
+            yr = icl::mfma_32x32x2_f32(dr, br, yr);
+            yr = icl::mfma_32x32x2_f32(-di, bi, yr);
+            yi = icl::mfma_32x32x2_f32(dr, bi, yi);
+            yi = icl::mfma_32x32x2_f32(di, br, yi);
+        } else { // lanes along rows
+            yr = icl::mfma_32x32x2_f32(br, dr, yr);
+            yr = icl::mfma_32x32x2_f32(bi, -di, yr);
+            yi = icl::mfma_32x32x2_f32(bi, dr, yi);
+            yi = icl::mfma_32x32x2_f32(br, di, yi);
+        }
+    }
+    const float2 sp = make_float2(a.s_post_re, a.s_post_im);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = wr + (LINES_CONTIG ? icl::mfma32_row(lane, r) : (lane & 31));
+        const int col = wc + (LINES_CONTIG ? (lane & 31) : icl::mfma32_row(lane, r));
+        if (row < a.nb && col < ncols)
+            X[(long)row * a.x_si + (long)(c0 + col) * a.x_sj] = c_mul(make_float2(yr[r], yi[r]), sp);
+    }
+}
+
+} // namespace
+
+extern "C" hipError_t icl_ctrsm_invert_blocks(const icl_ctrsm_blocked_args* a, hipStream_t stream)
+{
+    if (a->nsys <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((a->nsys + kNB - 1) / kNB);
+    hipLaunchKernelGGL(ctrsm_invert_kernel, dim3(blocks), dim3(64), 0, stream, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t icl_ctrsm_apply(const icl_ctrsm_apply_args* a, hipStream_t stream)
+{
+    if (a->nb <= 0 || a->nlines <= 0) return hipSuccess;
+    if (a->nb > kNB) return hipErrorInvalidValue;
+    static const hipError_t attr = [] {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(ctrsm_apply_kernel<true>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kApLds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(ctrsm_apply_kernel<false>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kApLds);
+        return e;
+    }();
+    if (attr != hipSuccess) return attr;
+    const unsigned blocks = (unsigned)((a->nlines + kNB - 1) / kNB);
+    if (a->x_sj == 1 || a->x_sj == -1)
+        hipLaunchKernelGGL(ctrsm_apply_kernel<true>, dim3(blocks), dim3(256), kApLds, stream, *a);
+    else
+        hipLaunchKernelGGL(ctrsm_apply_kernel<false>, dim3(blocks), dim3(256), kApLds, stream, *a);
+    return hipGetLastError();
+}

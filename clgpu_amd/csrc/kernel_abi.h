@@ -95,6 +95,32 @@ typedef struct icl_ctrsm_diag_args {
 } icl_ctrsm_diag_args;
 hipError_t icl_ctrsm_diag(const icl_ctrsm_diag_args* a, hipStream_t stream);
 
+/* Inverts the 64 x 64 diagonal blocks of the canonical complex system with nsys
+ * unknowns (L(i, k) = L[i * l_si + k * l_sk], interleaved complex, conjugated
+ * when conj) into invD: ceil(nsys/64) dense column-major 64 x 64 complex blocks. */
+typedef struct icl_ctrsm_blocked_args {
+    int nsys;
+    const float* L;
+    long l_si, l_sk;
+    int unit;
+    int conj;
+    float* invD;
+} icl_ctrsm_blocked_args;
+hipError_t icl_ctrsm_invert_blocks(const icl_ctrsm_blocked_args* a, hipStream_t stream);
+
+/* X_b = s_post * inv(L_bb) X_b in place for one diagonal block of nb <= 64
+ * unknowns and nlines lines: X(i, j) = X[i * x_si + j * x_sj] (interleaved
+ * complex), invD = that block's dense inverse. */
+typedef struct icl_ctrsm_apply_args {
+    int nb;
+    int nlines;
+    const float* invD;
+    float* X;
+    long x_si, x_sj;
+    float s_post_re, s_post_im;
+} icl_ctrsm_apply_args;
+hipError_t icl_ctrsm_apply(const icl_ctrsm_apply_args* a, hipStream_t stream);
+
 /* Blocked solve of the canonical lower-triangular system L X = B with nsys
  * unknowns and nlines right-hand sides: L(i, k) = L[i * l_si + k * l_sk],
  * B(i, j) = B[i * x_si + j * x_sj] (strides may be negative). The 64 x 64

@@ -74,7 +74,7 @@ def test_ctrsm_variants_vs_fp64(h, v, shape):
     order = m if side == 0 else n
     # within a small multiple of the reference's own float error, or of order*u
     assert err_gpu <= max(8 * err_cpu, 8 * order * 2.0 ** -24), (err_gpu, err_cpu)
-    assert h.last_implementation() == ("ctrsm_diag64" if order <= 64 else "ctrsm_recursive")
+    assert h.last_implementation() == ("ctrsm_diag64" if order <= 64 else "ctrsm_recursive_inv64")
 
 
 @pytest.mark.parametrize("v", VARIANTS[::5], ids=vid)
