@@ -75,9 +75,12 @@ static bool forced_is_gemm_of_kind(const std::string& name, int kind)
 
 const icl_gemm_variant* gemm_select(iclblasContext* h, const GemmProblem& p)
 {
+    // Problems using the level-3 fields run only on FP32 kernels (the split-BF16
+    // kernels do not implement them), so a forced split variant is not used there.
+    const bool level3 = p.args->write_tri != ICL_TRI_NONE || p.args->kband != ICL_KBAND_NONE || p.args->cflags != 0;
     if (!h->forced_impl.empty() && forced_is_gemm_of_kind(h->forced_impl, p.kind)) {
         for (const icl_gemm_variant* v : gemm_registry())
-            if (h->forced_impl == v->name) return v;
+            if (h->forced_impl == v->name && (!level3 || v->math == ICL_MATH_FP32)) return v;
     }
     const icl_gemm_variant* best = nullptr;
     float best_score = -1.0f;
@@ -107,6 +110,8 @@ static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, co
     if (math == ICL_MATH_BF16X3 && g.K < kSplitMinK) math = ICL_MATH_FP32;
     GemmProblem p{kind, math, opa, opb, &g};
     const icl_gemm_variant* v = gemm_select(h, p);
+    if ((g.write_tri != ICL_TRI_NONE || g.kband != ICL_KBAND_NONE || g.cflags != 0) && v->math != ICL_MATH_FP32)
+        throw status_error(ICLBLAS_STATUS_INTERNAL_ERROR, "level-3 GEMM options need an FP32 kernel");
     const bool edge = gemm_needs_edge(*v, p);
     void* ws = nullptr;
     if (v->workspace) {
@@ -169,6 +174,31 @@ void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int 
     g.beta_re = beta[0];
     g.beta_im = beta[1];
     run_gemm(h, ICL_KIND_CGEMM, math, opa, opb, g, true);
+}
+
+void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],
+                     const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc,
+                     int write_tri, int kband, int cflags)
+{
+    if (m <= 0 || n <= 0) return;
+    icl_gemm_args g{};
+    g.M = m;
+    g.N = n;
+    g.K = k;
+    g.A = A;
+    g.B = B;
+    g.C = C;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.ldc = ldc;
+    g.alpha_re = alpha[0];
+    g.alpha_im = kind == ICL_KIND_CGEMM ? alpha[1] : 0.0f;
+    g.beta_re = beta[0];
+    g.beta_im = kind == ICL_KIND_CGEMM ? beta[1] : 0.0f;
+    g.write_tri = write_tri;
+    g.kband = kband;
+    g.cflags = cflags;
+    run_gemm(h, kind, ICL_MATH_FP32, opa, opb, g, true);
 }
 
 // ---------------------------------------------------------------------------

@@ -42,6 +42,31 @@ void sgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int 
 void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int k, const float alpha[2],
                const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc);
 
+// One FP32 MFMA GEMM launch with the level-3 options of icl_gemm_args
+// (write_tri, kband, cflags). kind: ICL_KIND_SGEMM (alpha[1], beta[1] unused) or
+// ICL_KIND_CGEMM. k may be 0: C's selected triangle becomes beta * C and A, B are
+// not read.
+void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],
+                     const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc,
+                     int write_tri, int kband, int cflags);
+
+// GEMM-based level-3 routines on device pointers (complex data interleaved).
+// TRMM: C = alpha * op(A) * B (left) or alpha * B * op(A) (right), out of place;
+// C may alias B.
+void trmm_dev(iclblasContext* h, bool cplx, int side, int uplo, int trans, int diag, int m, int n,
+              const float alpha[2], const float* A, long lda, const float* B, long ldb, float* C, long ldc);
+// SYMM / HEMM: C = alpha * A * B + beta * C (left) or alpha * B * A + beta * C.
+void symm_dev(iclblasContext* h, bool cplx, bool herm, int side, int uplo, int m, int n, const float alpha[2],
+              const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc);
+// SYRK / HERK (herm: op is the conjugate transpose, alpha and beta real):
+// C = alpha * op(A) * op(A)^T|H + beta * C on the uplo triangle.
+void syrk_dev(iclblasContext* h, bool cplx, bool herm, int uplo, int trans, int n, int k, const float alpha[2],
+              const float* A, long lda, const float beta[2], float* C, long ldc);
+// SYR2K / HER2K: C = alpha op(A) op(B)^T + alpha op(B) op(A)^T + beta C, or for
+// herm alpha op(A) op(B)^H + conj(alpha) op(B) op(A)^H + beta C (beta real).
+void syr2k_dev(iclblasContext* h, bool cplx, bool herm, int uplo, int trans, int n, int k, const float alpha[2],
+               const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc);
+
 // Blocked STRSM, in place on B.
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
                const float* A, long lda, float* B, long ldb, bool record = true);

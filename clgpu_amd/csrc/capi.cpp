@@ -19,14 +19,13 @@
 #include <cstring>
 #include <new>
 
-#include "blas3.hpp"
-#include "handle.hpp"
+#include "capi_common.hpp"
 
 using icl::status_error;
 
-namespace {
+using namespace icl::capi;
 
-enum Slot { kSlotA = 0, kSlotB = 1, kSlotC = 2, kSlotW0 = 3, kSlotW1 = 4, kSlotW2 = 5 };
+namespace {
 
 // STRSM implementations above 64 unknowns (forceable by name): the panel solve
 // (default: 256 unknowns per launch, the rest updated by a GEMM), the
@@ -34,71 +33,6 @@ enum Slot { kSlotA = 0, kSlotB = 1, kSlotC = 2, kSlotW0 = 3, kSlotW1 = 4, kSlotW
 // dataflow solve, and the recursive split into diagonal-block solves plus GEMM
 // updates.
 const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_dataflow64", "strsm_panel256", "strsm_recursive"};
-
-template <typename F>
-iclblasStatus_t guarded(F&& f)
-{
-    try {
-        f();
-        return ICLBLAS_STATUS_SUCCESS;
-    } catch (const status_error& e) {
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        return ICLBLAS_STATUS_ALLOC_FAILED;
-    } catch (...) {
-        return ICLBLAS_STATUS_ERROR;
-    }
-}
-
-bool op_ok(int op) { return op == ICLBLAS_OP_N || op == ICLBLAS_OP_T || op == ICLBLAS_OP_C; }
-
-// Number of elements a column-major rows x cols matrix with leading dimension ld spans.
-size_t extent(int rows, int cols, int ld) { return rows == 0 || cols == 0 ? 0 : (size_t)(cols - 1) * ld + rows; }
-
-struct HandleLock {
-    iclblasContext* h;
-    explicit HandleLock(iclblasContext* hh) : h(hh)
-    {
-        h->mu.lock();
-        if (hipSetDevice(h->device) != hipSuccess) {
-            h->mu.unlock();
-            throw status_error(ICLBLAS_STATUS_EXECUTION_FAILED, "hipSetDevice");
-        }
-    }
-    ~HandleLock() { h->mu.unlock(); }
-};
-
-// Host-memory staging for the synchronous host-pointer mode: a matrix of
-// rows x cols with leading dimension ld is copied with a pitched copy so that
-// padding rows of the caller's buffer are never written.
-template <typename T>
-T* stage_in(iclblasContext* h, int slot, const T* host, int rows, int cols, int ld, bool copy)
-{
-    const size_t n = extent(rows, cols, ld);
-    T* d = static_cast<T*>(h->scratch[slot].get(n * sizeof(T), h->stream));
-    if (copy && n)
-        icl::hip_check(hipMemcpy2DAsync(d, (size_t)ld * sizeof(T), host, (size_t)ld * sizeof(T), (size_t)rows * sizeof(T),
-                                        (size_t)cols, hipMemcpyHostToDevice, h->stream),
-                       "hipMemcpy2DAsync(H2D)");
-    return d;
-}
-
-template <typename T>
-void stage_out(iclblasContext* h, T* host, const T* dev, int rows, int cols, int ld)
-{
-    if (rows == 0 || cols == 0) return;
-    icl::hip_check(hipMemcpy2DAsync(host, (size_t)ld * sizeof(T), dev, (size_t)ld * sizeof(T), (size_t)rows * sizeof(T),
-                                    (size_t)cols, hipMemcpyDeviceToHost, h->stream),
-                   "hipMemcpy2DAsync(D2H)");
-}
-
-void finish(iclblasContext* h)
-{
-    if (h->memory_mode == ICLBLAS_MEMORY_HOST) icl::hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
-    else icl::hip_check(hipGetLastError(), "launch");
-}
-
-int math_of(iclblasContext* h) { return h->math_mode == ICLBLAS_MATH_SPLIT_BF16X3 ? ICL_MATH_BF16X3 : ICL_MATH_FP32; }
 
 // Shared GEMM validation; *work is set when the call has work to do.
 iclblasStatus_t gemm_checks(iclblasHandle_t handle, int transa, int transb, int m, int n, int k, const void* alpha,

@@ -45,6 +45,34 @@ struct TriMask {
     __device__ __forceinline__ bool keep(int m, int n) const { return !on || (lower ? m - n : n - m) >= strict; }
 };
 
+// Level-3 tile window. Returns false when the tile [m0, m0 + BM) x [n0, n0 + BN)
+// lies wholly outside the triangle of C that g.write_tri selects (the whole
+// workgroup then exits before touching memory); otherwise sets [kb, ke), the k
+// range in which the triangular operand named by g.kband can be non-zero. kb is
+// m0 or n0, a multiple of BK because BM and BN are.
+template <int BM, int BN>
+__device__ __forceinline__ bool tile_window(const GemmArgs& g, int m0, int n0, int& kb, int& ke)
+{
+    if (g.write_tri == ICL_TRI_LOWER && m0 + BM - 1 < n0) return false;
+    if (g.write_tri == ICL_TRI_UPPER && m0 > n0 + BN - 1) return false;
+    kb = 0;
+    ke = g.K;
+    switch (g.kband) {
+    case ICL_KBAND_K_BELOW_M_END: ke = min(g.K, m0 + BM); break;
+    case ICL_KBAND_K_FROM_M0: kb = min(g.K, m0); break;
+    case ICL_KBAND_K_BELOW_N_END: ke = min(g.K, n0 + BN); break;
+    case ICL_KBAND_K_FROM_N0: kb = min(g.K, n0); break;
+    default: break;
+    }
+    return true;
+}
+
+// Store predicate of g.write_tri.
+__device__ __forceinline__ bool tri_writes(int write_tri, int m, int n)
+{
+    return write_tri == ICL_TRI_NONE || (write_tri == ICL_TRI_LOWER ? m >= n : m <= n);
+}
+
 // ---------------------------------------------------------------------------
 // Operand slab loaders.
 // An operand slab is ROWS x BK where "row" is m for op(A) and n for op(B).
@@ -396,7 +424,10 @@ struct CSplitSlab {
 // ---------------------------------------------------------------------------
 // Epilogue helpers
 // ---------------------------------------------------------------------------
-template <bool EDGE>
+// TRI_CHECK: honour g.write_tri (the FP32 kernels always test it; the uniform
+// branch also keeps the compiler from hoisting every store address of the tile
+// ahead of the stores, which otherwise spills registers).
+template <bool EDGE, bool TRI_CHECK = false>
 __device__ __forceinline__ void store_real_tile(const GemmArgs& g, const f32x16& acc, int mrow, int ncol0, int lane)
 {
     // mrow: m index of this lane; ncol0: n index of register 0's 32x32 block origin.
@@ -405,6 +436,8 @@ __device__ __forceinline__ void store_real_tile(const GemmArgs& g, const f32x16&
     for (int r = 0; r < 16; ++r) {
         const int n = ncol0 + mfma32_row(lane, r);
         if (EDGE && (mrow >= g.M || n >= g.N)) continue;
+        if constexpr (TRI_CHECK)
+            if (!tri_writes(g.write_tri, mrow, n)) continue;
         float* c = g.C + (long)n * g.ldc + mrow;
         float v = g.alpha_re * acc[r];
         if (g.beta_re != 0.0f) v = fmaf(g.beta_re, *c, v);
@@ -414,17 +447,21 @@ __device__ __forceinline__ void store_real_tile(const GemmArgs& g, const f32x16&
 
 // c = alpha * acc + beta * c with the complex product of the reference's
 // complex.h.cl:32-34 (cmul(a, b) = (fma(a.re, b.re, -a.im*b.im), fma(a.re, b.im, a.im*b.re))).
-template <bool EDGE>
+template <bool EDGE, bool L3 = false>
 __device__ __forceinline__ void store_complex_tile(const GemmArgs& g, const f32x16& accr, const f32x16& acci,
                                                    int mrow, int ncol0, int lane)
 {
     const bool beta_zero = (g.beta_re == 0.0f && g.beta_im == 0.0f);
+    const float conj_sign = (L3 && (g.cflags & ICL_CFLAG_CONJ_ACC)) ? -1.0f : 1.0f;
+    const bool real_diag = L3 && (g.cflags & ICL_CFLAG_REAL_DIAG) != 0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int n = ncol0 + mfma32_row(lane, r);
         if (EDGE && (mrow >= g.M || n >= g.N)) continue;
+        if constexpr (L3)
+            if (!tri_writes(g.write_tri, mrow, n)) continue;
         float* c = g.C + 2 * ((long)n * g.ldc + mrow);
-        const float xr = accr[r], xi = acci[r];
+        const float xr = accr[r], xi = conj_sign * acci[r];
         float vr = fmaf(g.alpha_re, xr, -g.alpha_im * xi);
         float vi = fmaf(g.alpha_re, xi, g.alpha_im * xr);
         if (!beta_zero) {
@@ -432,6 +469,7 @@ __device__ __forceinline__ void store_complex_tile(const GemmArgs& g, const f32x
             vr += fmaf(g.beta_re, old[0], -g.beta_im * old[1]);
             vi += fmaf(g.beta_re, old[1], g.beta_im * old[0]);
         }
+        if (real_diag && mrow == n) vi = 0.0f;
         *reinterpret_cast<f32x2*>(c) = f32x2{vr, vi};
     }
 }
@@ -452,7 +490,7 @@ struct SgemmF32 {
     static_assert(WM % 32 == 0 && WN % 32 == 0 && BK % 4 == 0, "bad tile");
 };
 
-template <int TA, int TB, int BM, int BN, int BK, int WAVES_M, int WAVES_N, bool EDGE>
+template <int TA, int TB, int BM, int BN, int BK, int WAVES_M, int WAVES_N, bool EDGE, bool L3 = false>
 __global__ void __launch_bounds__(WAVES_M* WAVES_N * 64)
 sgemm_f32_kernel(GemmArgs g)
 {
@@ -467,6 +505,14 @@ sgemm_f32_kernel(GemmArgs g)
     int tm, tn;
     tile_coords<8>((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
+    int kb = 0, ke = g.K;
+    if constexpr (L3)
+        if (!tile_window<BM, BN>(g, m0, n0, kb, ke)) return;
+    // The k window starts the operands at column / row kb; the loop below then
+    // runs over a K of ke - kb exactly as for a plain GEMM.
+    const float* const Aw = g.A + ((TA != 0) ? (long)kb : (long)kb * g.lda);
+    const float* const Bw = g.B + ((TB == 0) ? (long)kb : (long)kb * g.ldb);
+    const int Kw = ke - kb;
 
     typename Cfg::SlabA sa;
     typename Cfg::SlabB sb;
@@ -476,18 +522,18 @@ sgemm_f32_kernel(GemmArgs g)
 #pragma unroll
         for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = f32x16{};
 
-    const int KT = (g.K + BK - 1) / BK;
+    const int KT = (Kw + BK - 1) / BK;
     if (KT > 0) {
-        sa.load(g.A, g.lda, m0, 0, g.M, g.K, tid);
-        sb.load(g.B, g.ldb, n0, 0, g.N, g.K, tid);
+        sa.load(Aw, g.lda, m0, 0, g.M, Kw, tid);
+        sb.load(Bw, g.ldb, n0, 0, g.N, Kw, tid);
         sa.store(As(0), tid);
         sb.store(Bs(0), tid);
         __syncthreads();
         for (int kt = 0; kt < KT; ++kt) {
             const int cur = kt & 1;
             if (kt + 1 < KT) {
-                sa.load(g.A, g.lda, m0, (kt + 1) * BK, g.M, g.K, tid);
-                sb.load(g.B, g.ldb, n0, (kt + 1) * BK, g.N, g.K, tid);
+                sa.load(Aw, g.lda, m0, (kt + 1) * BK, g.M, Kw, tid);
+                sb.load(Bw, g.ldb, n0, (kt + 1) * BK, g.N, Kw, tid);
             }
             const float* as = As(cur);
             const float* bs = Bs(cur);
@@ -515,7 +561,7 @@ sgemm_f32_kernel(GemmArgs g)
     for (int i = 0; i < Cfg::FM; ++i)
 #pragma unroll
         for (int j = 0; j < Cfg::FN; ++j)
-            store_real_tile<EDGE>(g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
+            store_real_tile<EDGE, true>(g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -640,7 +686,7 @@ struct CgemmF32 {
     static constexpr size_t LDS_BYTES = 2 * STAGE_WORDS * sizeof(float);
 };
 
-template <int OA, int OB, int BM, int BN, int BK, int WAVES_M, int WAVES_N, bool EDGE>
+template <int OA, int OB, int BM, int BN, int BK, int WAVES_M, int WAVES_N, bool EDGE, bool L3 = false>
 __global__ void __launch_bounds__(WAVES_M* WAVES_N * 64)
 cgemm_f32_kernel(GemmArgs g)
 {
@@ -657,6 +703,14 @@ cgemm_f32_kernel(GemmArgs g)
     int tm, tn;
     tile_coords<8>((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
+    int kb = 0, ke = g.K;
+    if constexpr (L3)
+        if (!tile_window<BM, BN>(g, m0, n0, kb, ke)) return;
+    // The k window starts the operands at column / row kb; the loop below then
+    // runs over a K of ke - kb exactly as for a plain GEMM.
+    const float* const Aw = g.A + 2 * ((OA != 0) ? (long)kb : (long)kb * g.lda);
+    const float* const Bw = g.B + 2 * ((OB == 0) ? (long)kb : (long)kb * g.ldb);
+    const int Kw = ke - kb;
 
     typename Cfg::SlabA sa;
     typename Cfg::SlabB sb;
@@ -669,18 +723,18 @@ cgemm_f32_kernel(GemmArgs g)
             acci[i][j] = f32x16{};
         }
 
-    const int KT = (g.K + BK - 1) / BK;
+    const int KT = (Kw + BK - 1) / BK;
     if (KT > 0) {
-        sa.load(g.A, g.lda, m0, 0, g.M, g.K, tid);
-        sb.load(g.B, g.ldb, n0, 0, g.N, g.K, tid);
+        sa.load(Aw, g.lda, m0, 0, g.M, Kw, tid);
+        sb.load(Bw, g.ldb, n0, 0, g.N, Kw, tid);
         sa.store(Ar(0), Ai(0), tid);
         sb.store(Br(0), Bi(0), tid);
         __syncthreads();
         for (int kt = 0; kt < KT; ++kt) {
             const int cur = kt & 1;
             if (kt + 1 < KT) {
-                sa.load(g.A, g.lda, m0, (kt + 1) * BK, g.M, g.K, tid);
-                sb.load(g.B, g.ldb, n0, (kt + 1) * BK, g.N, g.K, tid);
+                sa.load(Aw, g.lda, m0, (kt + 1) * BK, g.M, Kw, tid);
+                sb.load(Bw, g.ldb, n0, (kt + 1) * BK, g.N, Kw, tid);
             }
             const float* ar_s = Ar(cur);
             const float* ai_s = Ai(cur);
@@ -729,7 +783,7 @@ cgemm_f32_kernel(GemmArgs g)
     for (int i = 0; i < Cfg::FM; ++i)
 #pragma unroll
         for (int j = 0; j < Cfg::FN; ++j)
-            store_complex_tile<EDGE>(g, accr[i][j], acci[i][j], m0 + wm0 + 32 * i + (lane & 31),
+            store_complex_tile<EDGE, L3>(g, accr[i][j], acci[i][j], m0 + wm0 + 32 * i + (lane & 31),
                                      n0 + wn0 + 32 * j, lane);
 }
 

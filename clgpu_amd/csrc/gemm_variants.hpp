@@ -14,11 +14,22 @@ struct VariantRegistrar {
 };
 
 // ---- SGEMM, FP32 MFMA ------------------------------------------------------
+// Problems that use the level-3 fields of icl_gemm_args run the kernel
+// instantiation that reads them; plain GEMMs run one compiled without them.
+inline bool uses_level3(const GemmArgs* g)
+{
+    return g->write_tri != ICL_TRI_NONE || g->kband != ICL_KBAND_NONE || g->cflags != 0;
+}
+
 template <int TA, int TB, int BM, int BN, int BK, int WVM, int WVN, bool EDGE>
 hipError_t run_sgemm_f32(const GemmArgs* g, hipStream_t s)
 {
     using Cfg = SgemmF32<TA, TB, BM, BN, BK, WVM, WVN, EDGE>;
-    return launch_gemm<BM, BN, Cfg::NT, Cfg::LDS_BYTES>(sgemm_f32_kernel<TA, TB, BM, BN, BK, WVM, WVN, EDGE>, g, s);
+    if (uses_level3(g))
+        return launch_gemm<BM, BN, Cfg::NT, Cfg::LDS_BYTES>(sgemm_f32_kernel<TA, TB, BM, BN, BK, WVM, WVN, EDGE, true>,
+                                                            g, s);
+    return launch_gemm<BM, BN, Cfg::NT, Cfg::LDS_BYTES>(sgemm_f32_kernel<TA, TB, BM, BN, BK, WVM, WVN, EDGE, false>, g,
+                                                        s);
 }
 
 // ---- SGEMM, split-BF16 x3 ----------------------------------------------------
@@ -67,10 +78,15 @@ hipError_t run_cgemm(const GemmArgs* g, int edge, hipStream_t s)
     } else {
         using C0 = CgemmF32<OA, OB, BM, BN, BK, WVM, WVN, false>;
         using C1 = CgemmF32<OA, OB, BM, BN, BK, WVM, WVN, true>;
-        return edge ? launch_gemm<BM, BN, C1::NT, C1::LDS_BYTES>(cgemm_f32_kernel<OA, OB, BM, BN, BK, WVM, WVN, true>,
-                                                                 g, s)
+        if (uses_level3(g))
+            return edge ? launch_gemm<BM, BN, C1::NT, C1::LDS_BYTES>(
+                              cgemm_f32_kernel<OA, OB, BM, BN, BK, WVM, WVN, true, true>, g, s)
+                        : launch_gemm<BM, BN, C0::NT, C0::LDS_BYTES>(
+                              cgemm_f32_kernel<OA, OB, BM, BN, BK, WVM, WVN, false, true>, g, s);
+        return edge ? launch_gemm<BM, BN, C1::NT, C1::LDS_BYTES>(
+                          cgemm_f32_kernel<OA, OB, BM, BN, BK, WVM, WVN, true, false>, g, s)
                     : launch_gemm<BM, BN, C0::NT, C0::LDS_BYTES>(
-                          cgemm_f32_kernel<OA, OB, BM, BN, BK, WVM, WVN, false>, g, s);
+                          cgemm_f32_kernel<OA, OB, BM, BN, BK, WVM, WVN, false, false>, g, s);
     }
 }
 

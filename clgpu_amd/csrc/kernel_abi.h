@@ -23,7 +23,32 @@ typedef struct icl_gemm_args {
     long lda, ldb, ldc; /* in elements of the data type */
     float alpha_re, alpha_im, beta_re, beta_im;
     int mask; /* output triangle mask, ICL_MASK_* */
+    /* Level-3 extensions, honoured by the FP32 MFMA kernels only (all zero for a
+     * plain GEMM): */
+    int write_tri; /* ICL_TRI_*: store only this triangle of C, leave the rest untouched */
+    int kband;     /* ICL_KBAND_*: per-tile k window of a zero-padded triangular operand */
+    int cflags;    /* ICL_CFLAG_*: complex epilogue options */
 } icl_gemm_args;
+
+enum {
+    ICL_TRI_NONE = 0,
+    ICL_TRI_LOWER = 1, /* m >= n */
+    ICL_TRI_UPPER = 2  /* m <= n */
+};
+
+/* k window of a tile at rows [m0, m0 + BM), columns [n0, n0 + BN): */
+enum {
+    ICL_KBAND_NONE = 0,
+    ICL_KBAND_K_BELOW_M_END = 1, /* op(A) lower triangular: k < m0 + BM */
+    ICL_KBAND_K_FROM_M0 = 2,     /* op(A) upper triangular: k >= m0 */
+    ICL_KBAND_K_BELOW_N_END = 3, /* op(B) upper triangular: k < n0 + BN */
+    ICL_KBAND_K_FROM_N0 = 4      /* op(B) lower triangular: k >= n0 */
+};
+
+enum {
+    ICL_CFLAG_CONJ_ACC = 1, /* C = alpha * conj(op(A) op(B)) + beta * C */
+    ICL_CFLAG_REAL_DIAG = 2 /* imaginary part of C(m, m) stored as 0 */
+};
 
 enum {
     ICL_MASK_NONE = 0,
@@ -199,6 +224,34 @@ hipError_t icl_axpby_2d(int m, int n, float alpha, const float* X, long ldx, flo
 /* Complex version of the same (used for CGEMM with k == 0 or alpha == 0). */
 hipError_t icl_caxpby_2d(int m, int n, float alpha_re, float alpha_im, const float* X, long ldx, float beta_re,
                          float beta_im, float* Y, long ldy, hipStream_t stream);
+
+/* Writes the dense order x order matrix W (column-major, leading dimension ldw)
+ * that a triangular or symmetric operand stands for, so that the level-3
+ * routines can run it through the GEMM kernels. A holds one triangle (lower if
+ * `lower`), the other triangle of A is never read.
+ *   ICL_EXPAND_TRI: W = op(A) with op 0 = N, 1 = T, 2 = conjugate transpose;
+ *                   entries outside the triangle are 0 and, if unit, the
+ *                   diagonal is 1 (the diagonal of A is then not read).
+ *   ICL_EXPAND_SYM: W(i, j) = A(i, j) inside the stored triangle and A(j, i)
+ *                   (conjugated if herm) outside it; herm also stores the
+ *                   diagonal's imaginary parts as 0.
+ * Complex data (icl_expand_complex) is interleaved {re, im}; lda/ldw count
+ * elements of the data type. */
+enum { ICL_EXPAND_TRI = 0, ICL_EXPAND_SYM = 1 };
+typedef struct icl_expand_args {
+    int order;
+    const float* A;
+    long lda;
+    float* W;
+    long ldw;
+    int mode;
+    int lower;
+    int trans;
+    int unit;
+    int herm;
+} icl_expand_args;
+hipError_t icl_expand_real(const icl_expand_args* a, hipStream_t stream);
+hipError_t icl_expand_complex(const icl_expand_args* a, hipStream_t stream);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -3,8 +3,9 @@
 Names, enum values and argument order are those of the reference's
 include/iclBLAS/iclBLAS.h (iclblasCreate :172, iclblasDestroy :179,
 iclblasSgemm :1685, iclblasStrsm :1719, iclblasCgemm :1799, iclblasCtrsm
-:1925) plus the
-extensions of include/iclBLAS_ext.h. Every function returns the C status;
+:1925; the level-3 routines Ssymm :1579, Ssyrk :1608, Ssyr2k :1645, Strmm
+:1753, Csymm :1825, Csyr2k :1862, Csyrk :1891, Cherk :1954, Cher2k :1991,
+Ctrmm :2025, Chemm :2051) plus the extensions of include/iclBLAS_ext.h. Every function returns the C status;
 `check()` turns a non-success status into IclblasError.
 
 Host-pointer mode (the default, as in the reference) takes numpy arrays in
@@ -74,6 +75,17 @@ _SIGS = {
     "iclblasStrsm": [_P, _I, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I],
     "iclblasCtrsm": [_P, _I, _I, _I, _I, _I, _I, _CP, _P, _I, _P, _I],
     "iclblasStrsmRefined": [_P, _I, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _I],
+    "iclblasSsymm": [_P, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _FP, _P, _I],
+    "iclblasCsymm": [_P, _I, _I, _I, _I, _CP, _P, _I, _P, _I, _CP, _P, _I],
+    "iclblasChemm": [_P, _I, _I, _I, _I, _CP, _P, _I, _P, _I, _CP, _P, _I],
+    "iclblasSsyrk": [_P, _I, _I, _I, _I, _FP, _P, _I, _FP, _P, _I],
+    "iclblasCsyrk": [_P, _I, _I, _I, _I, _CP, _P, _I, _CP, _P, _I],
+    "iclblasCherk": [_P, _I, _I, _I, _I, _FP, _P, _I, _FP, _P, _I],
+    "iclblasSsyr2k": [_P, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _FP, _P, _I],
+    "iclblasCsyr2k": [_P, _I, _I, _I, _I, _CP, _P, _I, _P, _I, _CP, _P, _I],
+    "iclblasCher2k": [_P, _I, _I, _I, _I, _CP, _P, _I, _P, _I, _FP, _P, _I],
+    "iclblasStrmm": [_P, _I, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _P, _I],
+    "iclblasCtrmm": [_P, _I, _I, _I, _I, _I, _I, _CP, _P, _I, _P, _I, _P, _I],
     "iclblasSgemmBackward": [_P, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _P, _I, _P, _I, _P, _I],
     "iclblasStrsmBackward": [_P, _I, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _P, _I, _P, _I, _P, _I],
     "iclblasSetMemoryMode": [_P, _I],
@@ -179,6 +191,60 @@ def iclblasCtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
 def iclblasStrsmRefined(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, max_sweeps):
     return load().iclblasStrsmRefined(handle, side, uplo, trans, diag, m, n, _fscalar(alpha), _ptr(A), lda, _ptr(B),
                                       ldb, max_sweeps)
+
+
+def iclblasSsymm(handle, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc):
+    return load().iclblasSsymm(handle, side, uplo, m, n, _fscalar(alpha), _ptr(A), lda, _ptr(B), ldb, _fscalar(beta),
+                               _ptr(C), ldc)
+
+
+def iclblasCsymm(handle, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc):
+    return load().iclblasCsymm(handle, side, uplo, m, n, _cscalar(alpha), _ptr(A), lda, _ptr(B), ldb, _cscalar(beta),
+                               _ptr(C), ldc)
+
+
+def iclblasChemm(handle, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc):
+    return load().iclblasChemm(handle, side, uplo, m, n, _cscalar(alpha), _ptr(A), lda, _ptr(B), ldb, _cscalar(beta),
+                               _ptr(C), ldc)
+
+
+def iclblasSsyrk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc):
+    return load().iclblasSsyrk(handle, uplo, trans, n, k, _fscalar(alpha), _ptr(A), lda, _fscalar(beta), _ptr(C), ldc)
+
+
+def iclblasCsyrk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc):
+    return load().iclblasCsyrk(handle, uplo, trans, n, k, _cscalar(alpha), _ptr(A), lda, _cscalar(beta), _ptr(C), ldc)
+
+
+def iclblasCherk(handle, uplo, trans, n, k, alpha, A, lda, beta, C, ldc):
+    """alpha and beta are real."""
+    return load().iclblasCherk(handle, uplo, trans, n, k, _fscalar(alpha), _ptr(A), lda, _fscalar(beta), _ptr(C), ldc)
+
+
+def iclblasSsyr2k(handle, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    return load().iclblasSsyr2k(handle, uplo, trans, n, k, _fscalar(alpha), _ptr(A), lda, _ptr(B), ldb, _fscalar(beta),
+                                _ptr(C), ldc)
+
+
+def iclblasCsyr2k(handle, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    return load().iclblasCsyr2k(handle, uplo, trans, n, k, _cscalar(alpha), _ptr(A), lda, _ptr(B), ldb, _cscalar(beta),
+                                _ptr(C), ldc)
+
+
+def iclblasCher2k(handle, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    """alpha is complex, beta real."""
+    return load().iclblasCher2k(handle, uplo, trans, n, k, _cscalar(alpha), _ptr(A), lda, _ptr(B), ldb, _fscalar(beta),
+                                _ptr(C), ldc)
+
+
+def iclblasStrmm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, C, ldc):
+    return load().iclblasStrmm(handle, side, uplo, trans, diag, m, n, _fscalar(alpha), _ptr(A), lda, _ptr(B), ldb,
+                               _ptr(C), ldc)
+
+
+def iclblasCtrmm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, C, ldc):
+    return load().iclblasCtrmm(handle, side, uplo, trans, diag, m, n, _cscalar(alpha), _ptr(A), lda, _ptr(B), ldb,
+                               _ptr(C), ldc)
 
 
 def iclblasSgemmBackward(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, dC, lddc, dA, ldda, dB, lddb):

@@ -8,7 +8,10 @@
  *
  * Implemented on the GPU: iclblasCreate, iclblasDestroy (iclBLAS.h:172,179),
  * iclblasSgemm (:1685), iclblasStrsm (:1719), iclblasCgemm (:1799),
- * iclblasCtrsm (:1925).
+ * iclblasCtrsm (:1925), and the GEMM-based level-3 routines iclblasSsymm
+ * (:1579), Ssyrk (:1608), Ssyr2k (:1645), Strmm (:1753), Csymm (:1825), Csyr2k
+ * (:1862), Csyrk (:1891), Cherk (:1954), Cher2k (:1991), Ctrmm (:2025), Chemm
+ * (:2051).
  * Every other entry point is exported for link compatibility and returns
  * ICLBLAS_STATUS_NOT_SUPPORTED, the status the reference reports for an
  * unimplemented function (libs/blas/shlib/iclBLASImpl.hpp:53-56).
@@ -124,6 +127,58 @@ ICLBLAS_API iclblasStatus_t iclblasCgemm(iclblasHandle_t handle, iclblasOperatio
  * alpha == 0: B is set to 0 and neither A nor B is read. */
 ICLBLAS_API iclblasStatus_t iclblasCtrsm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, iclblasOperation_t trans, iclblasDiagType_t diag, int m, int n, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb);
 
+/* ---- GEMM-based level-3 routines ---- */
+/* C = alpha*A*B + beta*C (left) or alpha*B*A + beta*C (right), A symmetric with
+ * only its uplo triangle read, B and C m x n (iclBLAS.h:1555-1579). beta == 0: C
+ * is not read. alpha == 0 and beta == 1: returns at once. */
+ICLBLAS_API iclblasStatus_t iclblasSsymm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, int m, int n, const float* alpha, float* A, int lda, float* B, int ldb, const float* beta, float* C, int ldc);
+
+/* C = alpha*op(A)*op(A)^T + beta*C on the uplo triangle of C; op(A) n x k, op in
+ * {N, T} (C means T) (iclBLAS.h:1581-1608). The other triangle is not touched.
+ * n == 0, k == 0, or alpha == 0 and beta == 1: returns at once. */
+ICLBLAS_API iclblasStatus_t iclblasSsyrk(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const float* alpha, float* A, int lda, const float* beta, float* C, int ldc);
+
+/* C = alpha*(op(A)*op(B)^T + op(B)*op(A)^T) + beta*C on the uplo triangle of C
+ * (iclBLAS.h:1610-1645); otherwise as iclblasSsyrk. */
+ICLBLAS_API iclblasStatus_t iclblasSsyr2k(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const float* alpha, float* A, int lda, float* B, int ldb, const float* beta, float* C, int ldc);
+
+/* C = alpha*op(A)*B (left) or alpha*B*op(A) (right), A triangular (uplo, diag),
+ * B and C m x n, C written and not read; C may be the same matrix as B
+ * (iclBLAS.h:1721-1753). alpha == 0: C = 0, A and B not read. */
+ICLBLAS_API iclblasStatus_t iclblasStrmm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, iclblasOperation_t transa, iclblasDiagType_t diag, int m, int n, const float* alpha, float* A, int lda, float* B, int ldb, float* C, int ldc);
+
+/* Complex symmetric (not hermitian) version of iclblasSsymm (iclBLAS.h:1801-1825). */
+ICLBLAS_API iclblasStatus_t iclblasCsymm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, int m, int n, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc);
+
+/* Complex symmetric rank-2k update C = alpha*(op(A)*op(B)^T + op(B)*op(A)^T) +
+ * beta*C on the uplo triangle; op C means op(X) = X^H, as the reference kernel
+ * computes it (iclBLAS.h:1827-1862). */
+ICLBLAS_API iclblasStatus_t iclblasCsyr2k(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc);
+
+/* Complex symmetric rank-k update C = alpha*op(A)*op(A)^T + beta*C on the uplo
+ * triangle; op C means op(A) = A^H, as the reference kernel computes it
+ * (iclBLAS.h:1864-1891). */
+ICLBLAS_API iclblasStatus_t iclblasCsyrk(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda, const oclComplex_t* beta, oclComplex_t* C, int ldc);
+
+/* C = alpha*op(A)*op(A)^H + beta*C on the uplo triangle, op in {N, C} (T means C),
+ * alpha and beta real; the diagonal's imaginary parts are stored as 0
+ * (iclBLAS.h:1927-1954). k == 0: C = beta*C on the triangle. */
+ICLBLAS_API iclblasStatus_t iclblasCherk(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const float* alpha, oclComplex_t* A, int lda, const float* beta, oclComplex_t* C, int ldc);
+
+/* C = alpha*op(A)*op(B)^H + conj(alpha)*op(B)*op(A)^H + beta*C on the uplo
+ * triangle, beta real; the diagonal's imaginary parts are stored as 0
+ * (iclBLAS.h:1956-1991). */
+ICLBLAS_API iclblasStatus_t iclblasCher2k(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, const float* beta, oclComplex_t* C, int ldc);
+
+/* Complex version of iclblasStrmm, op in {N, T, C} (iclBLAS.h:1993-2025). m <= 0,
+ * n <= 0 or uplo outside {0, 1}: returns SUCCESS at once, as the reference does. */
+ICLBLAS_API iclblasStatus_t iclblasCtrmm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, iclblasOperation_t transa, iclblasDiagType_t diag, int m, int n, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, oclComplex_t* C, int ldc);
+
+/* C = alpha*A*B + beta*C (left) or alpha*B*A + beta*C (right), A hermitian with
+ * only its uplo triangle read and the diagonal's imaginary parts taken as 0
+ * (iclBLAS.h:2027-2051). */
+ICLBLAS_API iclblasStatus_t iclblasChemm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, int m, int n, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc);
+
 /* ---- exported for link compatibility; each returns ICLBLAS_STATUS_NOT_SUPPORTED ---- */
 ICLBLAS_API iclblasStatus_t iclblasScopy(iclblasHandle_t handle, int n, float *x, int incx, float *y, int incy);
 ICLBLAS_API iclblasStatus_t iclblasSscal(iclblasHandle_t handle, int n, const float* alpha, float *x, int incx);
@@ -186,17 +241,6 @@ ICLBLAS_API iclblasStatus_t iclblasCtrsv(iclblasHandle_t handle, iclblasFillMode
 ICLBLAS_API iclblasStatus_t iclblasCtpsv(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, iclblasDiagType_t diag, int n, oclComplex_t* AP, oclComplex_t* x, int incx);
 ICLBLAS_API iclblasStatus_t iclblasChbmv(iclblasHandle_t handle, iclblasFillMode_t uplo, int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* x, int incx, const oclComplex_t* beta, oclComplex_t* y, int incy);
 ICLBLAS_API iclblasStatus_t iclblasCsyr(iclblasHandle_t handle, iclblasFillMode_t uplo, int n, const oclComplex_t* alpha, oclComplex_t* x, int incx, oclComplex_t* A, int lda);
-ICLBLAS_API iclblasStatus_t iclblasSsymm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, int m, int n, const float* alpha, float* A, int lda, float* B, int ldb, const float* beta, float* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasSsyrk(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const float* alpha, float* A, int lda, const float* beta, float* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasSsyr2k(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const float* alpha, float* A, int lda, float* B, int ldb, const float* beta, float* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasStrmm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, iclblasOperation_t transa, iclblasDiagType_t diag, int m, int n, const float* alpha, float* A, int lda, float* B, int ldb, float* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasCsymm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, int m, int n, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasCsyr2k(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasCsyrk(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda, const oclComplex_t* beta, oclComplex_t* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasCherk(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const float* alpha, oclComplex_t* A, int lda, const float* beta, oclComplex_t* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasCher2k(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans, int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, const float* beta, oclComplex_t* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasCtrmm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, iclblasOperation_t transa, iclblasDiagType_t diag, int m, int n, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, oclComplex_t* C, int ldc);
-ICLBLAS_API iclblasStatus_t iclblasChemm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, int m, int n, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -8,6 +8,8 @@ liboracle.so (oracle.c) restates the reference's CPU semantics:
   strsm        <- Strsm_naive, reference libs/blas/funcs/Strsm/Strsm_naive.cl:18-126
   ctrsm        <- Ctrsm_naive, reference libs/blas/funcs/Ctrsm/Ctrsm_naive.cl:16-183
 and provides FP64 references (dgemm, zgemm, dtrsm, ztrsm) for error reporting.
+level3.c adds FP64 oracles of the GEMM-based level-3 routines (symm / hemm,
+syrk / herk, syr2k / her2k, trmm; the sources it restates are listed there).
 _ref/libref_gemm.so, when present, is the reference's own cpu_gemm compiled from
 its headers (oracle/Makefile, target `ref`).
 
@@ -61,6 +63,10 @@ def lib():
         L.orc_dgemm_sample.argtypes = [I, I, I, D, P, I, P, I, D, P, I, I, P, P, P, P]
         L.orc_zgemm.argtypes = [I, I, I, I, I, P, P, I, P, I, P, P, I, P, P]
         L.orc_zgemm_sample.argtypes = [I, I, I, P, P, I, P, I, P, P, I, I, P, P, P, P]
+        L.orc_zsymm.argtypes = [I, I, I, I, I, P, P, I, P, I, P, P, I, P]
+        L.orc_zsyrk.argtypes = [I, I, I, I, I, P, P, I, P, P, I, P]
+        L.orc_zsyr2k.argtypes = [I, I, I, I, I, P, P, I, P, I, P, P, I, P]
+        L.orc_ztrmm.argtypes = [I, I, I, I, I, I, P, P, I, P, I, P, I, P]
         _lib = L
     return _lib
 
@@ -200,6 +206,64 @@ def ztrsm(side, uplo, trans, diag, m, n, alpha, A, B):
     al = np.array([complex(alpha).real, complex(alpha).imag], np.float64)
     lib().orc_ztrsm(side, uplo, trans, diag, m, n, _p(al), _p(A), _ld(A), _p(X), _ld(X))
     return X
+
+
+# ---------------------------------------------------------------- level 3 (FP64)
+def _z(X) -> np.ndarray:
+    """complex128 Fortran copy of a float32 / complex64 matrix."""
+    return np.asfortranarray(np.asarray(X, np.complex128))
+
+
+def _zs(v) -> np.ndarray:
+    v = complex(v)
+    return np.array([v.real, v.imag], np.float64)
+
+
+def _ret(Z, mag, m, n, real):
+    R = Z[:m, :n]
+    return (np.ascontiguousarray(R.real) if real else R), mag.reshape(n, m).T
+
+
+def zsymm(side, uplo, herm, m, n, alpha, A, B, beta, C):
+    """FP64 SYMM / HEMM. Returns (C result m x n, mag m x n); C is only read."""
+    real = not np.iscomplexobj(A)
+    Az, Bz, Cz = _z(A), _z(B), _z(C)
+    mag = np.zeros(m * n, np.float64)
+    lib().orc_zsymm(side, uplo, int(herm), m, n, _p(_zs(alpha)), _p(Az), _ld(Az), _p(Bz), _ld(Bz), _p(_zs(beta)),
+                    _p(Cz), _ld(Cz), _p(mag))
+    return _ret(Cz, mag, m, n, real)
+
+
+def zsyrk(uplo, trans, herm, n, k, alpha, A, beta, C):
+    """FP64 SYRK / HERK on the uplo triangle of C (the rest of C is returned as
+    given). Returns (C n x n, mag n x n)."""
+    real = not np.iscomplexobj(A)
+    Az, Cz = _z(A), _z(C)
+    mag = np.zeros(n * n, np.float64)
+    lib().orc_zsyrk(uplo, trans, int(herm), n, k, _p(_zs(alpha)), _p(Az), _ld(Az), _p(_zs(beta)), _p(Cz), _ld(Cz),
+                    _p(mag))
+    return _ret(Cz, mag, n, n, real)
+
+
+def zsyr2k(uplo, trans, herm, n, k, alpha, A, B, beta, C):
+    """FP64 SYR2K / HER2K on the uplo triangle of C. Returns (C n x n, mag n x n)."""
+    real = not np.iscomplexobj(A)
+    Az, Bz, Cz = _z(A), _z(B), _z(C)
+    mag = np.zeros(n * n, np.float64)
+    lib().orc_zsyr2k(uplo, trans, int(herm), n, k, _p(_zs(alpha)), _p(Az), _ld(Az), _p(Bz), _ld(Bz), _p(_zs(beta)),
+                     _p(Cz), _ld(Cz), _p(mag))
+    return _ret(Cz, mag, n, n, real)
+
+
+def ztrmm(side, uplo, trans, diag, m, n, alpha, A, B):
+    """FP64 TRMM (header contract). Returns (C m x n, mag m x n)."""
+    real = not np.iscomplexobj(A)
+    Az, Bz = _z(A), _z(B)
+    Cz = np.asfortranarray(np.zeros((m, n), np.complex128))
+    mag = np.zeros(m * n, np.float64)
+    lib().orc_ztrmm(side, uplo, trans, diag, m, n, _p(_zs(alpha)), _p(Az), _ld(Az), _p(Bz), _ld(Bz), _p(Cz),
+                    _ld(Cz), _p(mag))
+    return _ret(Cz, mag, m, n, real)
 
 
 # ---------------------------------------------------------------- error metrics

@@ -106,6 +106,51 @@ def test_cgemm_and_trsm_status_order(forged):
     assert ib.iclblasCtrsm(bad_handle, 0, 0, 2, 1, 2, 2, 1, z, 4, z, 4) == INV
 
 
+def test_level3_status_order(forged):
+    """The level-3 entry points' own early exits (reference iclBLAS_S.cpp:387-420,
+    512-520; iclBLAS_C.cpp:472-589), then the generic chain and the handle."""
+    bad_handle, _keep = forged
+    f = np.zeros((4, 4), np.float32, order="F")
+    z = np.zeros((4, 4), np.complex64, order="F")
+    INV, OK = ib.STATUS_INVALID_VALUE, ib.STATUS_SUCCESS
+    # SYMM / HEMM: m == 0 or n == 0 or (alpha == 0 and beta == 1) -> SUCCESS first
+    assert ib.iclblasSsymm(None, 0, 0, 0, -3, None, None, 0, None, 0, None, None, 0) == OK
+    assert ib.iclblasSsymm(bad_handle, 0, 0, 2, 2, 0.0, None, 0, None, 0, 1.0, None, 0) == OK
+    assert ib.iclblasChemm(bad_handle, 1, 1, 2, 2, 0j, None, 0, None, 0, 1 + 0j, None, 0) == OK
+    assert ib.iclblasCsymm(None, 0, 0, 2, 2, 0j, None, 0, None, 0, 1 + 1j, z, 4) == INV  # beta != 1: work
+    assert ib.iclblasSsymm(None, 0, 0, -1, 2, 1.0, f, 4, f, 4, 0.0, f, 4) == INV
+    assert ib.iclblasSsymm(None, 0, 0, 2, 2, None, f, 4, f, 4, 0.0, f, 4) == INV
+    assert ib.iclblasSsymm(None, 1, 0, 2, 5, 1.0, f, 4, f, 4, 0.0, f, 4) == INV  # lda < n (right side)
+    assert ib.iclblasSsymm(None, 0, 2, 2, 2, 1.0, f, 4, f, 4, 0.0, f, 4) == INV  # uplo out of range
+    assert ib.iclblasSsymm(bad_handle, 0, 0, 2, 2, 1.0, f, 4, f, 4, 0.0, f, 4) == INV
+    # SYRK / SYR2K (real and complex symmetric): n == 0 or k == 0 -> SUCCESS first
+    assert ib.iclblasSsyrk(None, 0, 0, 0, -1, None, None, 0, None, None, 0) == OK
+    assert ib.iclblasSsyrk(bad_handle, 0, 0, 3, 0, None, None, 0, None, None, 0) == OK
+    assert ib.iclblasCsyr2k(bad_handle, 0, 0, 2, 2, 0j, None, 0, None, 0, 1 + 0j, None, 0) == OK
+    assert ib.iclblasSsyrk(None, 0, 0, -2, 2, 1.0, f, 4, 0.0, f, 4) == INV
+    assert ib.iclblasSsyr2k(None, 0, 0, 2, 2, 1.0, f, 4, None, 4, 0.0, f, 4) == INV
+    assert ib.iclblasSsyrk(None, 0, 1, 2, 5, 1.0, f, 4, 0.0, f, 4) == INV  # op T: lda < k
+    assert ib.iclblasCsyrk(None, 0, 3, 2, 2, 1, z, 4, 0, z, 4) == INV  # trans out of range
+    assert ib.iclblasSsyrk(bad_handle, 1, 0, 2, 2, 1.0, f, 4, 0.0, f, 4) == INV
+    # CHERK / CHER2K: n < 0 or k < 0 -> INVALID_VALUE first; k == 0 is work
+    assert ib.iclblasCherk(None, 0, 0, 2, -1, None, None, 0, None, None, 0) == INV
+    assert ib.iclblasCher2k(None, 0, 0, -1, 2, None, None, 0, None, 0, None, None, 0) == INV
+    assert ib.iclblasCherk(bad_handle, 0, 0, 0, 2, None, None, 0, None, None, 0) == OK
+    assert ib.iclblasCherk(bad_handle, 0, 0, 2, 0, 1.0, z, 4, 1.0, z, 4) == INV
+    assert ib.iclblasCher2k(None, 0, 2, 2, 2, 1, z, 4, z, 4, None, z, 4) == INV
+    # STRMM: the generic chain only (n checked before m)
+    assert ib.iclblasStrmm(None, 0, 0, 0, 0, -1, 0, None, None, 0, None, 0, None, 0) == OK
+    assert ib.iclblasStrmm(None, 0, 0, 0, 0, 0, -1, None, None, 0, None, 0, None, 0) == INV
+    assert ib.iclblasStrmm(None, 0, 0, 0, 0, 2, 2, 1.0, f, 4, f, 4, None, 4) == INV
+    assert ib.iclblasStrmm(None, 0, 0, 0, 2, 2, 2, 1.0, f, 4, f, 4, f, 4) == INV  # diag out of range
+    assert ib.iclblasStrmm(bad_handle, 0, 0, 0, 0, 2, 2, 1.0, f, 4, f, 4, f, 4) == INV
+    # CTRMM: m <= 0, n <= 0 or uplo outside {0, 1} -> SUCCESS first (iclBLAS_C.cpp:568-569)
+    assert ib.iclblasCtrmm(None, 0, 0, 0, 0, -2, 2, None, None, 0, None, 0, None, 0) == OK
+    assert ib.iclblasCtrmm(bad_handle, 0, 5, 0, 0, 2, 2, 1, z, 4, z, 4, z, 4) == OK
+    assert ib.iclblasCtrmm(None, 0, 1, 0, 0, 2, 2, 1, z, 1, z, 4, z, 4) == INV  # lda < m
+    assert ib.iclblasCtrmm(bad_handle, 1, 1, 2, 1, 2, 2, 1, z, 4, z, 4, z, 4) == INV
+
+
 def test_handle_entry_points_reject_invalid_handles(forged):
     bad_handle, _keep = forged
     L = ib.load()
@@ -123,9 +168,9 @@ def test_unimplemented_entry_points_report_not_supported():
     f = L.iclblasSaxpy
     f.restype = ct.c_int
     assert f(None, 0, None, None, 1, None, 1) == ib.STATUS_NOT_SUPPORTED
-    g = L.iclblasCtrmm
+    g = L.iclblasSrotg
     g.restype = ct.c_int
-    assert g(None, 0, 0, 0, 0, 1, 1, None, None, 1, None, 1, None, 1) == ib.STATUS_NOT_SUPPORTED
+    assert g(None, None, None, None, None) == ib.STATUS_NOT_SUPPORTED
 
 
 def test_create_without_a_gpu_reports_not_initialized():
