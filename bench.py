@@ -45,9 +45,10 @@ def parse():
     ap.add_argument("--trans", default="NN", help="NN, NT, TN or TT for the timed op")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check (profiling runs)")
     ap.add_argument("--no-extra", action="store_true", help="skip the CGEMM / STRSM rows (same as --rows sgemm)")
-    ap.add_argument("--rows", default="sgemm,sgemm_more,cgemm,strsm,ctrsm,chain",
+    ap.add_argument("--rows", default="sgemm,sgemm_more,cgemm,strsm,ctrsm,chain,level3",
                     help="comma list of workloads to run: sgemm (the headline line), sgemm_more (the other SGEMM "
-                         "configurations), cgemm, strsm, ctrsm, chain (the differentiable GEMM+TRSM chain)")
+                         "configurations), cgemm, strsm, ctrsm, chain (the differentiable GEMM+TRSM chain), "
+                         "level3 (SYRK, SYR2K, SYMM, TRMM, HERK, HEMM, complex TRMM)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy kernel first (scales HBM byte counters)")
     return ap.parse_args()
@@ -178,6 +179,128 @@ def ctrsm_rows(h, ib, stream, steps: int) -> dict:
                       "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA, "forward_rel_err_fro": fe}
         del X, Xw, X64, opL
     del tT, tB, L64
+    torch.cuda.empty_cache()
+    return out
+
+
+def level3_rows(h, ib, stream, steps: int) -> dict:
+    """The GEMM-based level-3 routines on device-resident uniform[-1, 1] data
+    (torch's generator, seed 0), FP32 path: SSYRK, SSYR2K, SSYMM, STRMM at 8192,
+    CHERK, CHEMM, CTRMM at 4096. Useful flops (real flops; a complex
+    multiply-add counts 8): SYRK n^2 k, SYR2K 2 n^2 k, SYMM 2 m^2 n, TRMM m^2 n,
+    HERK 4 n^2 k, HEMM 8 m^2 n, complex TRMM 4 m n^2 (right side). Each result is
+    compared with an FP64 product on the GPU (normwise relative error over the
+    written part)."""
+    import torch
+    dev = torch.device("cuda:0")
+    h.set_math_mode(ib.MATH_FP32)
+    h.set_implementation(None)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+
+    def rnd(rows, cols, cplx=False):
+        """(cols, rows) tensor = column-major rows x cols matrix, ld = rows."""
+        x = torch.rand((cols, rows), generator=g, device=dev) * 2 - 1
+        if cplx:
+            x = torch.complex(x, torch.rand((cols, rows), generator=g, device=dev) * 2 - 1)
+        return x
+
+    def f64(x):  # the matrix (rows x cols) in float64 / complex128
+        return x.t().to(torch.complex128 if x.is_complex() else torch.float64)
+
+    def rel(got, exp, mask=None):
+        d = f64(got) - exp
+        if mask is not None:
+            d, exp = d * mask, exp * mask
+        return float(d.norm() / exp.norm())
+
+    def timed(call, flops):
+        call()
+        torch.cuda.synchronize()
+        ms = _time(stream, call, max(2, steps // 4), 1)
+        tf = flops / (ms * 1e-3) / 1e12
+        return {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
+                "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA}
+
+    out = {}
+    n = 8192
+    # SSYRK lower, op N: C = A A^T + 0.5 C
+    A, C = rnd(n, n), rnd(n, n)
+    C0 = C.clone()
+    call = lambda: ib.check(ib.iclblasSsyrk(h.value, 1, 0, n, n, 1.0, A.data_ptr(), n, 0.5, C.data_ptr(), n))
+    r = timed(call, float(n) * n * n)
+    C.copy_(C0)
+    call()
+    torch.cuda.synchronize()
+    low = torch.ones((n, n), device=dev, dtype=torch.float64).tril()
+    r["normwise_rel_err"] = rel(C, (f64(A) @ f64(A).t() + 0.5 * f64(C0)) * low, low)
+    out["ssyrk_8192_k8192_LN"] = r
+    # SSYR2K upper, op T, k = 4096
+    k = 4096
+    A2, B2 = rnd(k, n), rnd(k, n)
+    C.copy_(C0)
+    call = lambda: ib.check(ib.iclblasSsyr2k(h.value, 0, 1, n, k, 1.0, A2.data_ptr(), k, B2.data_ptr(), k, 0.0,
+                                             C.data_ptr(), n))
+    r = timed(call, 2.0 * n * n * k)
+    call()
+    torch.cuda.synchronize()
+    up = low.t()
+    P = f64(A2).t() @ f64(B2)
+    r["normwise_rel_err"] = rel(C, (P + P.t()) * up, up)
+    out["ssyr2k_8192_k4096_UT"] = r
+    del A2, B2, P
+    # SSYMM left, lower
+    Bm = rnd(n, n)
+    call = lambda: ib.check(ib.iclblasSsymm(h.value, 0, 1, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n, 0.0,
+                                            C.data_ptr(), n))
+    r = timed(call, 2.0 * n * n * n)
+    call()
+    torch.cuda.synchronize()
+    S = f64(A).tril()
+    S = S + S.tril(-1).t()
+    r["normwise_rel_err"] = rel(C, S @ f64(Bm))
+    out["ssymm_8192_LL"] = r
+    # STRMM left, lower, op N, non-unit (out of place)
+    call = lambda: ib.check(ib.iclblasStrmm(h.value, 0, 1, 0, 0, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n,
+                                            C.data_ptr(), n))
+    r = timed(call, float(n) * n * n)
+    call()
+    torch.cuda.synchronize()
+    r["normwise_rel_err"] = rel(C, f64(A).tril() @ f64(Bm))
+    out["strmm_8192_LLN"] = r
+    del A, C, C0, Bm, S, low, up
+    torch.cuda.empty_cache()
+
+    n = 4096
+    A, C, Bm = rnd(n, n, True), rnd(n, n, True), rnd(n, n, True)
+    # CHERK upper, op N, alpha 1, beta 0
+    call = lambda: ib.check(ib.iclblasCherk(h.value, 0, 0, n, n, 1.0, A.data_ptr(), n, 0.0, C.data_ptr(), n))
+    r = timed(call, 4.0 * n * n * n)
+    call()
+    torch.cuda.synchronize()
+    up = torch.ones((n, n), device=dev, dtype=torch.float64).triu().to(torch.complex128)
+    r["normwise_rel_err"] = rel(C, (f64(A) @ f64(A).conj().t()) * up, up)
+    out["cherk_4096_k4096_UN"] = r
+    # CHEMM left, upper
+    call = lambda: ib.check(ib.iclblasChemm(h.value, 0, 0, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n, 0.0,
+                                            C.data_ptr(), n))
+    r = timed(call, 8.0 * n * n * n)
+    call()
+    torch.cuda.synchronize()
+    Hm = f64(A).triu()
+    Hm = Hm + Hm.triu(1).conj().t()
+    Hm.diagonal().imag.zero_()
+    r["normwise_rel_err"] = rel(C, Hm @ f64(Bm))
+    out["chemm_4096_LU"] = r
+    # CTRMM right, upper, op C, non-unit
+    call = lambda: ib.check(ib.iclblasCtrmm(h.value, 1, 0, 2, 0, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n,
+                                            C.data_ptr(), n))
+    r = timed(call, 4.0 * n * n * n)
+    call()
+    torch.cuda.synchronize()
+    r["normwise_rel_err"] = rel(C, f64(Bm) @ f64(A).triu().conj().t())
+    out["ctrmm_4096_RUC"] = r
+    del A, C, Bm, Hm, up
     torch.cuda.empty_cache()
     return out
 
@@ -444,7 +567,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     rows = ["sgemm"] if a.no_extra else [r for r in a.rows.split(",") if r]
-    unknown = set(rows) - {"sgemm", "sgemm_more", "cgemm", "strsm", "ctrsm", "chain"}
+    unknown = set(rows) - {"sgemm", "sgemm_more", "cgemm", "strsm", "ctrsm", "chain", "level3"}
     if unknown:
         print(f"bench.py: unknown rows {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
@@ -491,6 +614,9 @@ def main():
     if "chain" in rows:
         print("[bench] GEMM+TRSM chain row", file=sys.stderr, flush=True)
         other["chain_2048x512"] = chain_rows(h, ib, stream, a.steps)
+    if "level3" in rows:
+        print("[bench] level-3 rows", file=sys.stderr, flush=True)
+        other["level3"] = level3_rows(h, ib, stream, a.steps)
     if other:
         line["other_rows"] = other
     if not a.no_cpu_baseline and A is not None:

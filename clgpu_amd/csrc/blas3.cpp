@@ -1,6 +1,7 @@
 // Host-side level-3 algorithms: implementation selection, GEMM dispatch, the
 // recursive blocked STRSM, mixed-precision refinement and the adjoints.
 #include "blas3.hpp"
+#include "level3_tiles.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -55,7 +56,10 @@ float gemm_score(const icl_gemm_variant& v, const GemmProblem& p)
     constexpr double kCUs = 256.0;
     const double macs = p.kind == ICL_KIND_CGEMM ? 4.0 : 1.0; // real MACs per element MAC
     const double conc = kCUs * (v.occupancy > 0 ? v.occupancy : 1);
-    const double tiles = double((g.M + v.bm - 1) / v.bm) * double((g.N + v.bn - 1) / v.bn);
+    // Level-3 launches: only the tiles meeting the written triangle run.
+    const double tiles = g.write_tri != ICL_TRI_NONE
+                             ? double(tri_tile_count(g.M, g.N, g.write_tri, v.bm, v.bn))
+                             : double((g.M + v.bm - 1) / v.bm) * double((g.N + v.bn - 1) / v.bn);
     const double rate = (v.tflops_hint > 0 ? v.tflops_hint : 1.0) * 1e12;
     const double t_tile = 2.0 * macs * v.bm * v.bn * double(g.K) * conc / rate;
     const double rounds = std::ceil(tiles / conc);

@@ -503,7 +503,11 @@ sgemm_f32_kernel(GemmArgs g)
     const int wm0 = (wave % WAVES_M) * Cfg::WM;
     const int wn0 = (wave / WAVES_M) * Cfg::WN;
     int tm, tn;
-    tile_coords<8>((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, tm, tn);
+    if constexpr (L3) {
+        if (!tile_coords_level3<BM, BN>(g.write_tri, g.kband, (g.M + BM - 1) / BM, (g.N + BN - 1) / BN, tm, tn))
+            return;
+    } else
+        tile_coords<8>((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     int kb = 0, ke = g.K;
     if constexpr (L3)
@@ -701,7 +705,11 @@ cgemm_f32_kernel(GemmArgs g)
     const int wm0 = (wave % WAVES_M) * Cfg::WM;
     const int wn0 = (wave / WAVES_M) * Cfg::WN;
     int tm, tn;
-    tile_coords<8>((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, tm, tn);
+    if constexpr (L3) {
+        if (!tile_coords_level3<BM, BN>(g.write_tri, g.kband, (g.M + BM - 1) / BM, (g.N + BN - 1) / BN, tm, tn))
+            return;
+    } else
+        tile_coords<8>((g.M + BM - 1) / BM, (g.N + BN - 1) / BN, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     int kb = 0, ke = g.K;
     if constexpr (L3)

@@ -4,7 +4,8 @@
 
 namespace icl {
 
-// Launches `kern(arg)` over ceil(M/BM) * ceil(N/BN) workgroups (M, N from g).
+// Launches `kern(arg)` over ceil(M/BM) * ceil(N/BN) workgroups (M, N from g),
+// or over the tiles that meet the written triangle when g->write_tri is set.
 // Kernels that need more than 64 KiB of LDS get the dynamic-LDS limit raised
 // once per instantiation.
 template <int BM, int BN, int NT, size_t LDS, typename Kern, typename Arg>
@@ -16,7 +17,8 @@ inline hipError_t launch_gemm(Kern kern, const GemmArgs* g, hipStream_t s, const
                                 static_cast<int>(LDS));
         if (attr != hipSuccess) return attr;
     }
-    const long tiles = (long)((g->M + BM - 1) / BM) * ((g->N + BN - 1) / BN);
+    const long tiles = g->write_tri != ICL_TRI_NONE ? tri_tile_count(g->M, g->N, g->write_tri, BM, BN)
+                                                    : (long)((g->M + BM - 1) / BM) * ((g->N + BN - 1) / BN);
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
     hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), LDS, s, arg);

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "level3_tiles.hpp"
+
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -92,6 +94,45 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, i
     const int r = b - g * per_group;
     tm = first + r % gsz;
     tn = r / gsz;
+}
+
+// Tile numbering of the level-3 launches (the grid holds only the tiles that
+// do work, so it must come from tri_tile_count on the host):
+//  - write_tri: the tiles meeting the written triangle, tile-row by tile-row;
+//    every XCD takes a contiguous share of that list (equal work per XCD,
+//    whereas contiguous bands of tile rows would load one XCD with most of the
+//    triangle);
+//  - kband: all tiles, longest k window first, so that the round-robin
+//    dispatch over the XCDs spreads the long and the short tiles evenly and
+//    the short ones fill the tail.
+// Returns false (the workgroup has no tile) only for ids beyond that count.
+template <int BM, int BN>
+__device__ __forceinline__ bool tile_coords_level3(int write_tri, int kband, int tiles_m, int tiles_n, int& tm,
+                                                   int& tn)
+{
+    const int nb = gridDim.x;
+    int b = blockIdx.x;
+    if (write_tri != 0) {
+        if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+        for (tm = 0; tm < tiles_m; ++tm) {
+            int lo, cnt;
+            tri_tile_row(write_tri, tm, tiles_n, BM, BN, lo, cnt);
+            if (b < cnt) {
+                tn = lo + b;
+                return true;
+            }
+            b -= cnt;
+        }
+        return false;
+    }
+    if (b >= tiles_m * tiles_n) return false;
+    switch (kband) {
+    case 1: tm = tiles_m - 1 - b / tiles_n; tn = b % tiles_n; break; // work grows with tm
+    case 2: tm = b / tiles_n; tn = b % tiles_n; break;               // work shrinks with tm
+    case 3: tn = tiles_n - 1 - b / tiles_m; tm = b % tiles_m; break; // work grows with tn
+    default: tn = b / tiles_m; tm = b % tiles_m; break;              // work shrinks with tn
+    }
+    return true;
 }
 
 } // namespace icl
