@@ -45,10 +45,10 @@ def parse():
     ap.add_argument("--trans", default="NN", help="NN, NT, TN or TT for the timed op")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check (profiling runs)")
     ap.add_argument("--no-extra", action="store_true", help="skip the CGEMM / STRSM rows (same as --rows sgemm)")
-    ap.add_argument("--rows", default="sgemm,sgemm_more,cgemm,strsm,ctrsm,chain,level3",
+    ap.add_argument("--rows", default="sgemm,sgemm_more,cgemm,strsm,ctrsm,chain,level3,gemv",
                     help="comma list of workloads to run: sgemm (the headline line), sgemm_more (the other SGEMM "
                          "configurations), cgemm, strsm, ctrsm, chain (the differentiable GEMM+TRSM chain), "
-                         "level3 (SYRK, SYR2K, SYMM, TRMM, HERK, HEMM, complex TRMM)")
+                         "level3 (SYRK, SYR2K, SYMM, TRMM, HERK, HEMM, complex TRMM), gemv (SGEMV, CGEMV)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy kernel first (scales HBM byte counters)")
     return ap.parse_args()
@@ -302,6 +302,44 @@ def level3_rows(h, ib, stream, steps: int) -> dict:
     out["ctrmm_4096_RUC"] = r
     del A, C, Bm, Hm, up
     torch.cuda.empty_cache()
+    return out
+
+
+def gemv_rows(h, ib, stream, steps: int) -> dict:
+    """SGEMV 16384 x 16384 (op N and T) and CGEMV 8192 x 8192 (op N and C) on
+    device-resident uniform[-1, 1] data, beta = 1. Algorithmic HBM bytes per
+    call: A once (m n elem), x once, y read and written; reported as GB/s
+    against the 8 TB/s HBM peak, with the error against an FP64 product."""
+    import torch
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    out = {}
+    for label, cplx, n, trans in (("sgemv_16384_N", False, 16384, 0), ("sgemv_16384_T", False, 16384, 1),
+                                  ("cgemv_8192_N", True, 8192, 0), ("cgemv_8192_C", True, 8192, 2)):
+        def rnd(*shape):
+            x = torch.rand(shape, generator=g, device=dev) * 2 - 1
+            return torch.complex(x, torch.rand(shape, generator=g, device=dev) * 2 - 1) if cplx else x
+        A, x, y = rnd(n, n), rnd(n), rnd(n)
+        y0 = y.clone()
+        fn = ib.iclblasCgemv if cplx else ib.iclblasSgemv
+        call = lambda: ib.check(fn(h.value, trans, n, n, 1.0, A.data_ptr(), n, x.data_ptr(), 1, 1.0, y.data_ptr(), 1))
+        call()
+        torch.cuda.synchronize()
+        ms = _time(stream, call, max(10, steps * 2), 2)
+        es = 8 if cplx else 4
+        nbytes = float(n) * n * es + 3.0 * n * es
+        y.copy_(y0)
+        call()
+        torch.cuda.synchronize()
+        M = A.t().to(torch.complex128 if cplx else torch.float64)
+        op = M if trans == 0 else (M.conj().t() if trans == 2 else M.t())
+        ref = op @ x.to(M.dtype) + y0.to(M.dtype)
+        out[label] = {"implementation": h.last_implementation(), "ms": ms, "GBps": nbytes / (ms * 1e-3) / 1e9,
+                      "frac_of_hbm_peak": nbytes / (ms * 1e-3) / 8e12,
+                      "normwise_rel_err": float((y.to(M.dtype) - ref).norm() / ref.norm())}
+        del A, x, y, y0, M, op, ref
+        torch.cuda.empty_cache()
     return out
 
 
@@ -567,7 +605,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     rows = ["sgemm"] if a.no_extra else [r for r in a.rows.split(",") if r]
-    unknown = set(rows) - {"sgemm", "sgemm_more", "cgemm", "strsm", "ctrsm", "chain", "level3"}
+    unknown = set(rows) - {"sgemm", "sgemm_more", "cgemm", "strsm", "ctrsm", "chain", "level3", "gemv"}
     if unknown:
         print(f"bench.py: unknown rows {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
@@ -617,6 +655,9 @@ def main():
     if "level3" in rows:
         print("[bench] level-3 rows", file=sys.stderr, flush=True)
         other["level3"] = level3_rows(h, ib, stream, a.steps)
+    if "gemv" in rows:
+        print("[bench] GEMV rows", file=sys.stderr, flush=True)
+        other["gemv"] = gemv_rows(h, ib, stream, a.steps)
     if other:
         line["other_rows"] = other
     if not a.no_cpu_baseline and A is not None:

@@ -67,6 +67,12 @@ void syrk_dev(iclblasContext* h, bool cplx, bool herm, int uplo, int trans, int 
 void syr2k_dev(iclblasContext* h, bool cplx, bool herm, int uplo, int trans, int n, int k, const float alpha[2],
                const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc);
 
+// GEMV: y = alpha * op(A) * x + beta * y on device pointers (complex data
+// interleaved; incx, incy >= 1). y is not read when beta == 0; alpha == 0 scales
+// y without reading A or x.
+void gemv_dev(iclblasContext* h, bool cplx, int trans, int m, int n, const float alpha[2], const float* A, long lda,
+              const float* x, long incx, const float beta[2], float* y, long incy);
+
 // Blocked STRSM, in place on B.
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
                const float* A, long lda, float* B, long ldb, bool record = true);

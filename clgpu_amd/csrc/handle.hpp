@@ -72,11 +72,12 @@ private:
 
 struct iclblasContext {
     static constexpr int kTag = 0xB1A5;
-    static constexpr int kScratchSlots = 10;
+    static constexpr int kScratchSlots = 11;
     static constexpr int kSlotTrsmWorkspace = 6; // inverted diagonal blocks + solution of blocked STRSM
     static constexpr int kSlotGemmWorkspace = 7; // packed operands of split-BF16 GEMMs
     static constexpr int kSlotExpanded = 8;      // dense copy of a triangular / symmetric operand (level 3)
     static constexpr int kSlotLevel3Out = 9;     // TRMM result when C overlaps B in device memory mode
+    static constexpr int kSlotGemvPartial = 10;  // split partial sums of GEMV
 
     int tag = kTag;
     int device = 0;

@@ -225,6 +225,30 @@ hipError_t icl_axpby_2d(int m, int n, float alpha, const float* X, long ldx, flo
 hipError_t icl_caxpby_2d(int m, int n, float alpha_re, float alpha_im, const float* X, long ldx, float beta_re,
                          float beta_im, float* Y, long ldy, hipStream_t stream);
 
+/* y = alpha * op(A) * x + beta * y (op 0 = N, 1 = T, 2 = conjugate transpose for
+ * complex), A m x n column-major; y is not read when beta == 0. icl_sgemv works on
+ * floats, icl_cgemv on interleaved complex (lda, incx, incy in elements of the
+ * type). splits > 1 splits the reduction dimension over the grid; partial then
+ * holds splits * len(y) elements (len(y) = m for op N, n otherwise), summed by a
+ * second kernel in split order. icl_gemv_splits gives the split count the
+ * launch is sized for. incx, incy >= 1. */
+typedef struct icl_gemv_args {
+    int trans;
+    int m, n;
+    const float* A;
+    long lda;
+    const float* x;
+    long incx;
+    float* y;
+    long incy;
+    float alpha_re, alpha_im, beta_re, beta_im;
+    float* partial;
+    int splits;
+} icl_gemv_args;
+int icl_gemv_splits(int cplx, int trans, int m, int n);
+hipError_t icl_sgemv(const icl_gemv_args* a, hipStream_t stream);
+hipError_t icl_cgemv(const icl_gemv_args* a, hipStream_t stream);
+
 /* Writes the dense order x order matrix W (column-major, leading dimension ldw)
  * that a triangular or symmetric operand stands for, so that the level-3
  * routines can run it through the GEMM kernels. A holds one triangle (lower if

@@ -5,7 +5,8 @@ include/iclBLAS/iclBLAS.h (iclblasCreate :172, iclblasDestroy :179,
 iclblasSgemm :1685, iclblasStrsm :1719, iclblasCgemm :1799, iclblasCtrsm
 :1925; the level-3 routines Ssymm :1579, Ssyrk :1608, Ssyr2k :1645, Strmm
 :1753, Csymm :1825, Csyr2k :1862, Csyrk :1891, Cherk :1954, Cher2k :1991,
-Ctrmm :2025, Chemm :2051) plus the extensions of include/iclBLAS_ext.h. Every function returns the C status;
+Ctrmm :2025, Chemm :2051; Sgemv :1049, Cgemv :1164) plus the extensions of
+include/iclBLAS_ext.h. Every function returns the C status;
 `check()` turns a non-success status into IclblasError.
 
 Host-pointer mode (the default, as in the reference) takes numpy arrays in
@@ -75,6 +76,8 @@ _SIGS = {
     "iclblasStrsm": [_P, _I, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I],
     "iclblasCtrsm": [_P, _I, _I, _I, _I, _I, _I, _CP, _P, _I, _P, _I],
     "iclblasStrsmRefined": [_P, _I, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _I],
+    "iclblasSgemv": [_P, _I, _I, _I, _FP, _P, _I, _P, _I, _FP, _P, _I],
+    "iclblasCgemv": [_P, _I, _I, _I, _CP, _P, _I, _P, _I, _CP, _P, _I],
     "iclblasSsymm": [_P, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _FP, _P, _I],
     "iclblasCsymm": [_P, _I, _I, _I, _I, _CP, _P, _I, _P, _I, _CP, _P, _I],
     "iclblasChemm": [_P, _I, _I, _I, _I, _CP, _P, _I, _P, _I, _CP, _P, _I],
@@ -191,6 +194,16 @@ def iclblasCtrsm(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb):
 def iclblasStrsmRefined(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, max_sweeps):
     return load().iclblasStrsmRefined(handle, side, uplo, trans, diag, m, n, _fscalar(alpha), _ptr(A), lda, _ptr(B),
                                       ldb, max_sweeps)
+
+
+def iclblasSgemv(handle, trans, m, n, alpha, A, lda, x, incx, beta, y, incy):
+    return load().iclblasSgemv(handle, trans, m, n, _fscalar(alpha), _ptr(A), lda, _ptr(x), incx, _fscalar(beta),
+                               _ptr(y), incy)
+
+
+def iclblasCgemv(handle, trans, m, n, alpha, A, lda, x, incx, beta, y, incy):
+    return load().iclblasCgemv(handle, trans, m, n, _cscalar(alpha), _ptr(A), lda, _ptr(x), incx, _cscalar(beta),
+                               _ptr(y), incy)
 
 
 def iclblasSsymm(handle, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc):

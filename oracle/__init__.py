@@ -9,7 +9,8 @@ liboracle.so (oracle.c) restates the reference's CPU semantics:
   ctrsm        <- Ctrsm_naive, reference libs/blas/funcs/Ctrsm/Ctrsm_naive.cl:16-183
 and provides FP64 references (dgemm, zgemm, dtrsm, ztrsm) for error reporting.
 level3.c adds FP64 oracles of the GEMM-based level-3 routines (symm / hemm,
-syrk / herk, syr2k / her2k, trmm; the sources it restates are listed there).
+syrk / herk, syr2k / her2k, trmm) and level2.c one of gemv (the sources they
+restate are listed there).
 _ref/libref_gemm.so, when present, is the reference's own cpu_gemm compiled from
 its headers (oracle/Makefile, target `ref`).
 
@@ -67,6 +68,7 @@ def lib():
         L.orc_zsyrk.argtypes = [I, I, I, I, I, P, P, I, P, P, I, P]
         L.orc_zsyr2k.argtypes = [I, I, I, I, I, P, P, I, P, I, P, P, I, P]
         L.orc_ztrmm.argtypes = [I, I, I, I, I, I, P, P, I, P, I, P, I, P]
+        L.orc_zgemv.argtypes = [I, I, I, P, P, I, P, I, P, P, I, P, P]
         _lib = L
     return _lib
 
@@ -264,6 +266,22 @@ def ztrmm(side, uplo, trans, diag, m, n, alpha, A, B):
     lib().orc_ztrmm(side, uplo, trans, diag, m, n, _p(_zs(alpha)), _p(Az), _ld(Az), _p(Bz), _ld(Bz), _p(Cz),
                     _ld(Cz), _p(mag))
     return _ret(Cz, mag, m, n, real)
+
+
+def zgemv(trans, m, n, alpha, A, x, incx, beta, y, incy):
+    """FP64 GEMV (level2.c). A is a float32 / complex64 Fortran matrix, x and y
+    1-D arrays read with strides incx / incy. Returns (y result of length len(y)
+    as float64 / complex128, mag)."""
+    real = not np.iscomplexobj(A)
+    Az = _z(A)
+    xz = np.ascontiguousarray(np.asarray(x, np.complex128))
+    yz = np.ascontiguousarray(np.asarray(y, np.complex128))
+    leny = m if trans == 0 else n
+    out = np.zeros(leny, np.complex128)
+    mag = np.zeros(leny, np.float64)
+    lib().orc_zgemv(trans, m, n, _p(_zs(alpha)), _p(Az), _ld(Az), _p(xz), incx, _p(_zs(beta)), _p(yz), incy, _p(out),
+                    _p(mag))
+    return (out.real.copy() if real else out), mag
 
 
 # ---------------------------------------------------------------- error metrics

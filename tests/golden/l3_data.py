@@ -1,6 +1,7 @@
 """Input data of the reference's known-answer tests that are built by loops
 rather than written out, rebuilt here with the same float32 arithmetic.
 
+sgemv_arrays: tests/iclBLAS/test_Sgemv.cpp (all 8 cases).
 ssymm_256: tests/iclBLAS/test_Ssymm.cpp:236-277 (256x256_left_up).
   A (lda = 256) is filled with 1.2f, then the upper triangle set to
   (float)(i*j + i + j) / (m*m + 2*m); B[i] = 1.f * i / ldb / n; C = 2.3f;
@@ -34,3 +35,18 @@ def ssymm_256_arrays():
     B = np.asfortranarray(Bflat.reshape(n, ldb).T)
     C = np.full((ldc, n), f32(2.3), np.float32, order="F")
     return A, B, C
+
+
+def _ratio(count, n):
+    """v[i] = 1.f * i / n / n for i < count, in float32 left to right."""
+    return (np.arange(count).astype(np.float32) / f32(n)) / f32(n)
+
+
+def sgemv_arrays(c):
+    """test_Sgemv.cpp: A[i], x[i], y[i] = 1.f * i / n / n over A (lda * n), x
+    (n * incx for op N, m * incx otherwise) and y (m * incy for op N, n * incy
+    otherwise), with the test's n. Returns (A as a Fortran lda x n matrix, x, y)."""
+    m, n, lda = c["m"], c["n"], c["lda"]
+    lenx, leny = (n, m) if c["trans"] == 0 else (m, n)
+    A = np.asfortranarray(_ratio(lda * n, n).reshape(n, lda).T)
+    return A, _ratio(lenx * c["incx"], n), _ratio(leny * c["incy"], n)
