@@ -151,6 +151,31 @@ def test_level3_status_order(forged):
     assert ib.iclblasCtrmm(bad_handle, 1, 1, 2, 1, 2, 2, 1, z, 4, z, 4, z, 4) == INV
 
 
+def test_gemv_status_order(forged):
+    """iclBLAS_S.cpp:477-479 / iclBLAS_C.cpp:238-242, then iclBLASImpl.hpp:129-166
+    (x, incx, y, incy, A, lda), then the handle."""
+    bad_handle, _keep = forged
+    f = np.zeros((4, 4), np.float32, order="F")
+    v = np.zeros(8, np.float32)
+    z = np.zeros((4, 4), np.complex64, order="F")
+    w = np.zeros(8, np.complex64)
+    INV, OK, NS = ib.STATUS_INVALID_VALUE, ib.STATUS_SUCCESS, ib.STATUS_NOT_SUPPORTED
+    assert ib.iclblasSgemv(None, 0, 0, -1, None, None, 0, None, 0, None, None, 0) == OK
+    assert ib.iclblasSgemv(bad_handle, 0, 2, 2, 0.0, None, 0, None, 0, 1.0, None, 0) == OK
+    assert ib.iclblasCgemv(bad_handle, 2, 2, 2, 0j, None, 0, None, 0, 1 + 0j, None, 0) == OK
+    assert ib.iclblasSgemv(None, 0, -1, 2, 1.0, f, 4, v, 1, 0.0, v, 1) == INV
+    assert ib.iclblasSgemv(None, 0, 2, 2, 1.0, f, 4, v, 0, 0.0, v, 1) == INV
+    assert ib.iclblasSgemv(None, 0, 2, 2, 1.0, f, 4, None, 1, 0.0, v, 1) == INV
+    assert ib.iclblasSgemv(None, 0, 2, 2, 1.0, f, 4, v, -1, 0.0, v, 1) == NS
+    assert ib.iclblasSgemv(None, 0, 2, 2, 1.0, f, 4, v, 1, 0.0, None, 1) == INV
+    assert ib.iclblasCgemv(None, 0, 2, 2, 1, z, 4, w, 1, 0, w, -2) == NS
+    assert ib.iclblasCgemv(None, 0, 2, 2, 1, None, 4, w, 1, 0, w, 1) == INV
+    assert ib.iclblasSgemv(None, 0, 5, 2, 1.0, f, 4, v, 1, 0.0, v, 1) == INV  # lda < m
+    assert ib.iclblasSgemv(None, 3, 2, 2, 1.0, f, 4, v, 1, 0.0, v, 1) == INV  # trans out of range
+    assert ib.iclblasSgemv(None, 0, 2, 2, None, f, 4, v, 1, 0.0, v, 1) == INV
+    assert ib.iclblasCgemv(bad_handle, 2, 2, 2, 1, z, 4, w, 1, 0, w, 1) == INV
+
+
 def test_handle_entry_points_reject_invalid_handles(forged):
     bad_handle, _keep = forged
     L = ib.load()
