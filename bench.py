@@ -45,10 +45,11 @@ def parse():
     ap.add_argument("--trans", default="NN", help="NN, NT, TN or TT for the timed op")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check (profiling runs)")
     ap.add_argument("--no-extra", action="store_true", help="skip the CGEMM / STRSM rows (same as --rows sgemm)")
-    ap.add_argument("--rows", default="sgemm,sgemm_more,cgemm,strsm,ctrsm,chain,level3,gemv",
+    ap.add_argument("--rows", default="sgemm,sgemm_more,cgemm,strsm,ctrsm,chain,level3,gemv,host",
                     help="comma list of workloads to run: sgemm (the headline line), sgemm_more (the other SGEMM "
                          "configurations), cgemm, strsm, ctrsm, chain (the differentiable GEMM+TRSM chain), "
-                         "level3 (SYRK, SYR2K, SYMM, TRMM, HERK, HEMM, complex TRMM), gemv (SGEMV, CGEMV)")
+                         "level3 (SYRK, SYR2K, SYMM, TRMM, HERK, HEMM, complex TRMM), gemv (SGEMV, CGEMV), "
+                         "host (host-pointer calls, PCIe included)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy kernel first (scales HBM byte counters)")
     return ap.parse_args()
@@ -343,6 +344,60 @@ def gemv_rows(h, ib, stream, steps: int) -> dict:
     return out
 
 
+def host_rows(ib, steps: int) -> dict:
+    """The reference's calling convention: host pointers, synchronous calls, so
+    every call moves its matrices over PCIe (through the handle's pinned
+    buffers). Wall-clock time per call on numpy arrays: SGEMM 8192^3 NN (FP32 and
+    split-BF16; A and B in, C out, beta = 0 so C is not sent), STRSM 4096 x 1024
+    (A and B in, B out) and SGEMV 16384^2 (A in). These PCIe-inclusive rates are
+    reported beside the device-resident ones, never as the headline value."""
+    import time
+    rng = np.random.default_rng(0)
+    out = {}
+
+    def wall(fn, k):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        return (time.perf_counter() - t0) / k
+
+    k = max(2, steps // 4)
+    with ib.Handle() as h:
+        n = 8192
+        A = np.asfortranarray(rng.uniform(-1, 1, (n, n)).astype(np.float32))
+        B = np.asfortranarray(rng.uniform(-1, 1, (n, n)).astype(np.float32))
+        C = np.asfortranarray(np.zeros((n, n), np.float32))
+        moved = 3.0 * n * n * 4
+        for label, mode in (("sgemm_8192_fp32", ib.MATH_FP32), ("sgemm_8192_split", ib.MATH_SPLIT_BF16X3)):
+            h.set_math_mode(mode)
+            t = wall(lambda: ib.check(ib.iclblasSgemm(h.value, 0, 0, n, n, n, 1.0, A, n, B, n, 0.0, C, n)), k)
+            out[label] = {"ms": t * 1e3, "tflops": 2.0 * n ** 3 / t / 1e12, "pcie_bytes": moved,
+                          "pcie_GBps_if_all_transfer": moved / t / 1e9, "implementation": h.last_implementation()}
+        h.set_math_mode(ib.MATH_FP32)
+        m, nr = 4096, 1024
+        T = np.asfortranarray(np.tril(rng.uniform(-1, 1, (m, m)) / 64 + np.diag(rng.uniform(1, 2, m))).astype(np.float32))
+        Bt = np.asfortranarray(rng.uniform(-1, 1, (m, nr)).astype(np.float32))
+        X = Bt.copy(order="F")
+
+        def trsm():
+            X[:] = Bt
+            ib.check(ib.iclblasStrsm(h.value, 0, 1, 0, 0, m, nr, 1.0, T, m, X, m))
+        t = wall(trsm, k)
+        moved = (m * m + 2.0 * m * nr) * 4
+        out["strsm_4096x1024"] = {"ms": t * 1e3, "tflops": float(m) * m * nr / t / 1e12, "pcie_bytes": moved,
+                                  "pcie_GBps_if_all_transfer": moved / t / 1e9}
+        del T, Bt, X
+        n = 16384
+        Ag = np.asfortranarray(rng.uniform(-1, 1, (n, n)).astype(np.float32))
+        x = rng.uniform(-1, 1, n).astype(np.float32)
+        y = np.zeros(n, np.float32)
+        t = wall(lambda: ib.check(ib.iclblasSgemv(h.value, 0, n, n, 1.0, Ag, n, x, 1, 0.0, y, 1)), k)
+        moved = (n * n + 2.0 * n) * 4
+        out["sgemv_16384"] = {"ms": t * 1e3, "pcie_bytes": moved, "pcie_GBps_if_all_transfer": moved / t / 1e9}
+    return out
+
+
 def chain_rows(h, ib, stream, steps: int) -> dict:
     """The differentiable chain X = A^-1 (alpha G H), loss = sum(W * X) with A
     2048^2 lower triangular, G 2048^2, H 2048 x 512, alpha = 0.5: one step is the
@@ -605,7 +660,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     rows = ["sgemm"] if a.no_extra else [r for r in a.rows.split(",") if r]
-    unknown = set(rows) - {"sgemm", "sgemm_more", "cgemm", "strsm", "ctrsm", "chain", "level3", "gemv"}
+    unknown = set(rows) - {"sgemm", "sgemm_more", "cgemm", "strsm", "ctrsm", "chain", "level3", "gemv", "host"}
     if unknown:
         print(f"bench.py: unknown rows {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
@@ -658,6 +713,9 @@ def main():
     if "gemv" in rows:
         print("[bench] GEMV rows", file=sys.stderr, flush=True)
         other["gemv"] = gemv_rows(h, ib, stream, a.steps)
+    if "host" in rows:
+        print("[bench] host-pointer rows", file=sys.stderr, flush=True)
+        other["host_pointer"] = host_rows(ib, a.steps)
     if other:
         line["other_rows"] = other
     if not a.no_cpu_baseline and A is not None:

@@ -50,18 +50,18 @@ struct HandleLock {
     ~HandleLock() { h->mu.unlock(); }
 };
 
-// Host-memory staging for the synchronous host-pointer mode: a matrix of
-// rows x cols with leading dimension ld is copied with a pitched copy so that
-// padding rows of the caller's buffer are never written.
+// Host-memory staging for the synchronous host-pointer mode, through the
+// handle's pinned buffers. In: the whole extent of a rows x cols matrix with
+// leading dimension ld (padding included; it lies inside the caller's
+// allocation) goes to a device scratch buffer in one pass. Out: only the rows
+// of each column are written back, so padding rows of the caller's buffer are
+// never written.
 template <typename T>
 T* stage_in(iclblasContext* h, int slot, const T* host, int rows, int cols, int ld, bool copy)
 {
     const size_t n = extent(rows, cols, ld);
     T* d = static_cast<T*>(h->scratch[slot].get(n * sizeof(T), h->stream));
-    if (copy && n)
-        icl::hip_check(hipMemcpy2DAsync(d, (size_t)ld * sizeof(T), host, (size_t)ld * sizeof(T), (size_t)rows * sizeof(T),
-                                        (size_t)cols, hipMemcpyHostToDevice, h->stream),
-                       "hipMemcpy2DAsync(H2D)");
+    if (copy && n) h->stager.h2d(d, host, n * sizeof(T), h->stream);
     return d;
 }
 
@@ -69,9 +69,7 @@ template <typename T>
 void stage_out(iclblasContext* h, T* host, const T* dev, int rows, int cols, int ld)
 {
     if (rows == 0 || cols == 0) return;
-    icl::hip_check(hipMemcpy2DAsync(host, (size_t)ld * sizeof(T), dev, (size_t)ld * sizeof(T), (size_t)rows * sizeof(T),
-                                    (size_t)cols, hipMemcpyDeviceToHost, h->stream),
-                   "hipMemcpy2DAsync(D2H)");
+    h->stager.d2h(host, dev, rows, cols, ld, sizeof(T), h->stream);
 }
 
 inline void finish(iclblasContext* h)

@@ -68,6 +68,35 @@ private:
     size_t cap_ = 0;
 };
 
+// Host <-> device copies of the host-pointer memory mode through pinned
+// memory. Pageable copies run at about half the PCIe rate, and pitched ones on
+// pageable memory are split into one transfer per column, so instead the
+// handle keeps two pinned buffers (hipHostMalloc, grown once and reused) and
+// pipelines chunks: while the DMA engine moves one buffer, CPU threads fill or
+// drain the other. Declared here, defined in staging.cpp.
+class PinnedStager {
+public:
+    PinnedStager() = default;
+    PinnedStager(const PinnedStager&) = delete;
+    PinnedStager& operator=(const PinnedStager&) = delete;
+    ~PinnedStager() { release(); }
+
+    // dst (device) <- the `bytes` at src (host), enqueued on `stream`. Returns once
+    // src may be reused (its bytes are in pinned memory or on the device).
+    void h2d(void* dst, const void* src, size_t bytes, hipStream_t stream);
+    // Column-major host matrix (rows x cols, leading dimension ld, es bytes per
+    // element) <- the same layout on the device after the work queued on
+    // `stream`; rows ld > rows of the host matrix are not written. Synchronous.
+    void d2h(void* dst, const void* src, int rows, int cols, long ld, size_t es, hipStream_t stream);
+    void release();
+
+private:
+    void ensure(size_t bytes);
+    void* buf_[2] = {nullptr, nullptr};
+    hipEvent_t done_[2] = {nullptr, nullptr}; // DMA of the chunk last placed in buf_[i] finished
+    size_t cap_ = 0;
+};
+
 } // namespace icl
 
 struct iclblasContext {
@@ -89,6 +118,7 @@ struct iclblasContext {
     std::string last_impl;
     std::mutex mu; // a handle serializes its own calls
     icl::DeviceScratch scratch[kScratchSlots];
+    icl::PinnedStager stager;
     int* trsm_error = nullptr; // device word set by an aborted dataflow STRSM (allocated on first use)
 
     static bool valid(const iclblasContext* h) { return h != nullptr && h->tag == kTag; }
