@@ -152,64 +152,94 @@ __global__ void __launch_bounds__(64) ctrsm_invert_kernel(icl_ctrsm_blocked_args
 
 // X_b = s_post * inv(L_bb) B_b for one diagonal block and all lines, in place:
 // B(i, j) = X[i * x_si + j * x_sj] (interleaved complex, i < nb). One workgroup
-// per 64 lines; each wave forms a 32 x 32 quadrant of the product with four
-// real v_mfma_f32_32x32x2_f32 chains (Yr = Dr Br - Di Bi, Yi = Dr Bi + Di Br).
+// of two waves per 32 lines (so that a 1024-line solve spreads over 32 CUs);
+// wave w forms rows 32w..32w+31 of the product with v_mfma_f32_32x32x2_f32,
+// Yr = Dr Br - Di Bi and Yi = Dr Bi + Di Br, the k range split in two halves
+// with their own accumulators (four independent MFMA chains per wave).
 // LINES_CONTIG: the lines are the contiguous dimension of B (right-side
 // solves), so the result's lanes run along lines for coalesced stores.
+constexpr int kApLines = 32;
+constexpr int kApThreads = 128;
 constexpr int kApLd = kNB + 1;
-constexpr size_t kApLds = sizeof(float) * 4 * kNB * kApLd;
+constexpr int kApLdB = kApLines + 1;
+constexpr size_t kApLds = sizeof(float) * (2 * kNB * kApLd + 2 * kNB * kApLdB);
 template <bool LINES_CONTIG>
-__global__ void __launch_bounds__(256) ctrsm_apply_kernel(icl_ctrsm_apply_args a)
+__global__ void __launch_bounds__(kApThreads) ctrsm_apply_kernel(icl_ctrsm_apply_args a)
 {
     extern __shared__ float sm[];
     float(*Dr)[kApLd] = reinterpret_cast<float(*)[kApLd]>(sm); // D[k][r] = inv(L_bb)(r, k)
     float(*Di)[kApLd] = Dr + kNB;
-    float(*Br)[kApLd] = Di + kNB; // B[k][l] = B_b(k, l)
-    float(*Bi)[kApLd] = Br + kNB;
+    float(*Br)[kApLdB] = reinterpret_cast<float(*)[kApLdB]>(sm + 2 * kNB * kApLd); // B[k][l] = B_b(k, l)
+    float(*Bi)[kApLdB] = Br + kNB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int c0 = blockIdx.x * kNB;
-    const int ncols = min(kNB, a.nlines - c0);
+    const int c0 = blockIdx.x * kApLines;
+    const int ncols = min(kApLines, a.nlines - c0);
     const float2* inv = reinterpret_cast<const float2*>(a.invD);
     float2* X = reinterpret_cast<float2*>(a.X);
-    for (int e = tid; e < kNB * kNB; e += 256) { // inv is dense column-major: e = k * 64 + r
-        const float2 v = inv[e];
-        Dr[e / kNB][e % kNB] = v.x;
-        Di[e / kNB][e % kNB] = v.y;
+    // All global reads are issued before the first LDS store, so their latencies
+    // overlap instead of adding up.
+    constexpr int kInvPer = kNB * kNB / kApThreads, kBPer = kNB * kApLines / kApThreads;
+    float2 iv[kInvPer], bv[kBPer];
+#pragma unroll
+    for (int i = 0; i < kInvPer; ++i) iv[i] = inv[tid + i * kApThreads]; // dense column-major: e = k * 64 + r
+#pragma unroll
+    for (int i = 0; i < kBPer; ++i) {
+        const int e = tid + i * kApThreads;
+        const int k = LINES_CONTIG ? e / kApLines : e % kNB, l = LINES_CONTIG ? e % kApLines : e / kNB;
+        const bool in = k < a.nb && l < ncols;
+        const long off = in ? (long)k * a.x_si + (long)(c0 + l) * a.x_sj : 0;
+        bv[i] = in ? X[off] : make_float2(0.0f, 0.0f);
     }
-    for (int e = tid; e < kNB * kNB; e += 256) {
-        const int k = LINES_CONTIG ? e / kNB : e % kNB, l = LINES_CONTIG ? e % kNB : e / kNB;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (k < a.nb && l < ncols) v = X[(long)k * a.x_si + (long)(c0 + l) * a.x_sj];
-        Br[k][l] = v.x;
-        Bi[k][l] = v.y;
+#pragma unroll
+    for (int i = 0; i < kInvPer; ++i) {
+        const int e = tid + i * kApThreads;
+        Dr[e / kNB][e % kNB] = iv[i].x;
+        Di[e / kNB][e % kNB] = iv[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < kBPer; ++i) {
+        const int e = tid + i * kApThreads;
+        const int k = LINES_CONTIG ? e / kApLines : e % kNB, l = LINES_CONTIG ? e % kApLines : e / kNB;
+        Br[k][l] = bv[i].x;
+        Bi[k][l] = bv[i].y;
     }
     __syncthreads();
-    const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
-    f32x16 yr = f32x16{}, yi = f32x16{};
+    const int wr = 32 * wave;
+    f32x16 yr0 = f32x16{}, yi0 = f32x16{}, yr1 = f32x16{}, yi1 = f32x16{};
 #pragma unroll 4
-    for (int kk = 0; kk < kNB; kk += 2) {
-        const int kr = kk + (lane >> 5);
-        const float dr = Dr[kr][wr + (lane & 31)], di = Di[kr][wr + (lane & 31)];
-        const float br = Br[kr][wc + (lane & 31)], bi = Bi[kr][wc + (lane & 31)];
+    for (int kk = 0; kk < kNB / 2; kk += 2) {
+        const int k0 = kk + (lane >> 5), k1 = k0 + kNB / 2;
+        const int rr = wr + (lane & 31), ll = lane & 31;
+        const float dr0 = Dr[k0][rr], di0 = Di[k0][rr], br0 = Br[k0][ll], bi0 = Bi[k0][ll];
+        const float dr1 = Dr[k1][rr], di1 = Di[k1][rr], br1 = Br[k1][ll], bi1 = Bi[k1][ll];
         if (LINES_CONTIG) { // lanes along lines
-            yr = icl::mfma_32x32x2_f32(dr, br, yr);
-            yr = icl::mfma_32x32x2_f32(-di, bi, yr);
-            yi = icl::mfma_32x32x2_f32(dr, bi, yi);
-            yi = icl::mfma_32x32x2_f32(di, br, yi);
+            yr0 = icl::mfma_32x32x2_f32(dr0, br0, yr0);
+            yi0 = icl::mfma_32x32x2_f32(dr0, bi0, yi0);
+            yr1 = icl::mfma_32x32x2_f32(dr1, br1, yr1);
+            yi1 = icl::mfma_32x32x2_f32(dr1, bi1, yi1);
+            yr0 = icl::mfma_32x32x2_f32(-di0, bi0, yr0);
+            yi0 = icl::mfma_32x32x2_f32(di0, br0, yi0);
+            yr1 = icl::mfma_32x32x2_f32(-di1, bi1, yr1);
+            yi1 = icl::mfma_32x32x2_f32(di1, br1, yi1);
         } else { // lanes along rows
-            yr = icl::mfma_32x32x2_f32(br, dr, yr);
-            yr = icl::mfma_32x32x2_f32(bi, -di, yr);
-            yi = icl::mfma_32x32x2_f32(bi, dr, yi);
-            yi = icl::mfma_32x32x2_f32(br, di, yi);
+            yr0 = icl::mfma_32x32x2_f32(br0, dr0, yr0);
+            yi0 = icl::mfma_32x32x2_f32(bi0, dr0, yi0);
+            yr1 = icl::mfma_32x32x2_f32(br1, dr1, yr1);
+            yi1 = icl::mfma_32x32x2_f32(bi1, dr1, yi1);
+            yr0 = icl::mfma_32x32x2_f32(bi0, -di0, yr0);
+            yi0 = icl::mfma_32x32x2_f32(br0, di0, yi0);
+            yr1 = icl::mfma_32x32x2_f32(bi1, -di1, yr1);
+            yi1 = icl::mfma_32x32x2_f32(br1, di1, yi1);
         }
     }
     const float2 sp = make_float2(a.s_post_re, a.s_post_im);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int row = wr + (LINES_CONTIG ? icl::mfma32_row(lane, r) : (lane & 31));
-        const int col = wc + (LINES_CONTIG ? (lane & 31) : icl::mfma32_row(lane, r));
+        const int col = LINES_CONTIG ? (lane & 31) : icl::mfma32_row(lane, r);
         if (row < a.nb && col < ncols)
-            X[(long)row * a.x_si + (long)(c0 + col) * a.x_sj] = c_mul(make_float2(yr[r], yi[r]), sp);
+            X[(long)row * a.x_si + (long)(c0 + col) * a.x_sj] =
+                c_mul(make_float2(yr0[r] + yr1[r], yi0[r] + yi1[r]), sp);
     }
 }
 
@@ -236,10 +266,10 @@ extern "C" hipError_t icl_ctrsm_apply(const icl_ctrsm_apply_args* a, hipStream_t
         return e;
     }();
     if (attr != hipSuccess) return attr;
-    const unsigned blocks = (unsigned)((a->nlines + kNB - 1) / kNB);
+    const unsigned blocks = (unsigned)((a->nlines + kApLines - 1) / kApLines);
     if (a->x_sj == 1 || a->x_sj == -1)
-        hipLaunchKernelGGL(ctrsm_apply_kernel<true>, dim3(blocks), dim3(256), kApLds, stream, *a);
+        hipLaunchKernelGGL(ctrsm_apply_kernel<true>, dim3(blocks), dim3(kApThreads), kApLds, stream, *a);
     else
-        hipLaunchKernelGGL(ctrsm_apply_kernel<false>, dim3(blocks), dim3(256), kApLds, stream, *a);
+        hipLaunchKernelGGL(ctrsm_apply_kernel<false>, dim3(blocks), dim3(kApThreads), kApLds, stream, *a);
     return hipGetLastError();
 }
