@@ -653,6 +653,10 @@ void strsm_refined_dev(iclblasContext* h, int side, int uplo, int trans, int dia
     const bool left = side == ICLBLAS_SIDE_LEFT;
     const bool tr = trans != ICLBLAS_OP_N;
     const bool lower_eff = (uplo == ICLBLAS_FILL_MODE_LOWER) != tr;
+    float* part = static_cast<float*>(
+        h->scratch[iclblasContext::kSlotRefine].get(2 * ICL_ABSMAX_BLOCKS * sizeof(float), h->stream));
+    float host_part[2 * ICL_ABSMAX_BLOCKS];
+    h->last_sweeps = 0;
     for (int s = 0; s < sweeps; ++s) {
         icl_residual_args r{};
         r.M = m;
@@ -686,6 +690,21 @@ void strsm_refined_dev(iclblasContext* h, int side, int uplo, int trans, int dia
         // D = op(A)^-1 R (in place in R), X += D
         strsm_dev(h, side, uplo, trans, diag, m, n, 1.0f, A, lda, R, m, false);
         hip_check(icl_axpby_2d(m, n, 1.0f, R, m, 1.0f, B, ldb, h->stream), "axpby");
+        h->last_sweeps = s + 1;
+        if (s + 1 == sweeps) break;
+        // Converged when the correction no longer changes X at FP32 resolution:
+        // max |D| <= 2^-24 max |X|. Further sweeps would recompute the same
+        // residual up to rounding, so the loop stops early.
+        hip_check(icl_absmax_pair(m, n, R, m, B, ldb, part, h->stream), "absmax");
+        hip_check(hipMemcpyAsync(host_part, part, sizeof(host_part), hipMemcpyDeviceToHost, h->stream),
+                  "hipMemcpyAsync(D2H)");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        float md = 0.0f, mx = 0.0f;
+        for (int b = 0; b < ICL_ABSMAX_BLOCKS; ++b) {
+            md = std::max(md, host_part[2 * b]);
+            mx = std::max(mx, host_part[2 * b + 1]);
+        }
+        if (md <= 0x1p-24f * mx) break;
     }
     h->last_impl = "strsm_refined_f64res";
 }

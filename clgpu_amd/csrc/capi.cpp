@@ -365,6 +365,14 @@ ICLBLAS_API iclblasStatus_t iclblasGetLastImplementation(iclblasHandle_t handle,
     return ICLBLAS_STATUS_SUCCESS;
 }
 
+ICLBLAS_API iclblasStatus_t iclblasGetLastRefinementSweeps(iclblasHandle_t handle, int* sweeps)
+{
+    if (!iclblasContext::valid(handle) || !sweeps) return ICLBLAS_STATUS_INVALID_VALUE;
+    std::lock_guard<std::mutex> g(handle->mu);
+    *sweeps = handle->last_sweeps;
+    return ICLBLAS_STATUS_SUCCESS;
+}
+
 // ---------------------------------------------------------------------------
 // Adjoints
 // ---------------------------------------------------------------------------

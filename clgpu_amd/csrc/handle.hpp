@@ -101,12 +101,13 @@ private:
 
 struct iclblasContext {
     static constexpr int kTag = 0xB1A5;
-    static constexpr int kScratchSlots = 11;
+    static constexpr int kScratchSlots = 12;
     static constexpr int kSlotTrsmWorkspace = 6; // inverted diagonal blocks + solution of blocked STRSM
     static constexpr int kSlotGemmWorkspace = 7; // packed operands of split-BF16 GEMMs
     static constexpr int kSlotExpanded = 8;      // dense copy of a triangular / symmetric operand (level 3)
     static constexpr int kSlotLevel3Out = 9;     // TRMM result when C overlaps B in device memory mode
     static constexpr int kSlotGemvPartial = 10;  // split partial sums of GEMV
+    static constexpr int kSlotRefine = 11;       // per-block maxima of the refinement convergence test
 
     int tag = kTag;
     int device = 0;
@@ -116,6 +117,7 @@ struct iclblasContext {
     iclblasMathMode_t math_mode = ICLBLAS_MATH_FP32;
     std::string forced_impl;
     std::string last_impl;
+    int last_sweeps = 0; // refinement sweeps the last iclblasStrsmRefined ran
     std::mutex mu; // a handle serializes its own calls
     icl::DeviceScratch scratch[kScratchSlots];
     icl::PinnedStager stager;

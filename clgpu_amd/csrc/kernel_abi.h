@@ -217,6 +217,14 @@ typedef struct icl_residual_args {
 } icl_residual_args;
 hipError_t icl_residual_f64(const icl_residual_args* a, hipStream_t stream);
 
+/* Refinement convergence test: partial[2 b] = max |D| and partial[2 b + 1] =
+ * max |X| over block b's share of the m x n matrices D and X (column-major,
+ * leading dimensions ldd, ldx), for ICL_ABSMAX_BLOCKS blocks; the host takes the
+ * maxima of the partials. NaNs are ignored. */
+#define ICL_ABSMAX_BLOCKS 256
+hipError_t icl_absmax_pair(int m, int n, const float* D, long ldd, const float* X, long ldx, float* partial,
+                           hipStream_t stream);
+
 /* Y(m x n) = alpha * X + beta * Y elementwise (beta == 0: Y not read;
  * X may be NULL when alpha == 0). */
 hipError_t icl_axpby_2d(int m, int n, float alpha, const float* X, long ldx, float beta, float* Y, long ldy,

@@ -929,6 +929,46 @@ extern "C" hipError_t icl_residual_f64(const icl_residual_args* a, hipStream_t s
     return hipGetLastError();
 }
 
+namespace {
+
+__global__ void __launch_bounds__(256) absmax_pair_kernel(int m, int n, const float* D, long ldd, const float* X,
+                                                          long ldx, float* partial)
+{
+    __shared__ float sd[256], sx[256];
+    const long total = (long)m * n;
+    float md = 0.0f, mx = 0.0f;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const long j = e / m, i = e - j * m;
+        md = fmaxf(md, fabsf(D[j * ldd + i]));
+        mx = fmaxf(mx, fabsf(X[j * ldx + i]));
+    }
+    sd[threadIdx.x] = md;
+    sx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sd[threadIdx.x] = fmaxf(sd[threadIdx.x], sd[threadIdx.x + s]);
+            sx[threadIdx.x] = fmaxf(sx[threadIdx.x], sx[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sd[0];
+        partial[2 * blockIdx.x + 1] = sx[0];
+    }
+}
+
+} // namespace
+
+extern "C" hipError_t icl_absmax_pair(int m, int n, const float* D, long ldd, const float* X, long ldx, float* partial,
+                                      hipStream_t stream)
+{
+    if (m <= 0 || n <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(absmax_pair_kernel, dim3(ICL_ABSMAX_BLOCKS), dim3(256), 0, stream, m, n, D, ldd, X, ldx,
+                       partial);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t icl_axpby_2d(int m, int n, float alpha, const float* X, long ldx, float beta, float* Y,
                                    long ldy, hipStream_t stream)
 {

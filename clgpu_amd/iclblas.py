@@ -100,6 +100,7 @@ _SIGS = {
     "iclblasSynchronize": [_P],
     "iclblasSetImplementation": [_P, ct.c_char_p],
     "iclblasGetLastImplementation": [_P, ct.c_char_p, _I],
+    "iclblasGetLastRefinementSweeps": [_P, ct.POINTER(_I)],
     "iclblasGetImplementations": [ct.c_char_p, _I],
 }
 
@@ -325,6 +326,11 @@ class Handle:
         buf = ct.create_string_buffer(128)
         check(load().iclblasGetLastImplementation(self.h, buf, 128), "iclblasGetLastImplementation")
         return buf.value.decode()
+
+    def last_refinement_sweeps(self) -> int:
+        v = ct.c_int()
+        check(load().iclblasGetLastRefinementSweeps(self.h, ct.byref(v)), "iclblasGetLastRefinementSweeps")
+        return v.value
 
 
 def exported_symbols() -> list[str]:

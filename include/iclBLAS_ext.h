@@ -64,11 +64,16 @@ ICLBLAS_API iclblasStatus_t iclblasGetImplementations(char* buf, int len);
 /* STRSM with mixed-precision iterative refinement: X0 = FP32 solve; then for
  * each sweep R = alpha*B - op(A)*X in FP64 on the GPU, D = FP32 solve of R,
  * X += D. Same arguments and validation as iclblasStrsm plus
- * 0 <= max_sweeps <= 8. max_sweeps == 0 is the plain solve. */
+ * 0 <= max_sweeps <= 8. max_sweeps == 0 is the plain solve. The sweeps stop
+ * early once a correction no longer changes X at FP32 resolution
+ * (max |D| <= 2^-24 max |X|); that test waits for the handle's stream once per
+ * sweep, also in the device-pointer memory mode. */
 ICLBLAS_API iclblasStatus_t iclblasStrsmRefined(iclblasHandle_t handle, iclblasSideMode_t side,
                                                 iclblasFillMode_t uplo, iclblasOperation_t trans,
                                                 iclblasDiagType_t diag, int m, int n, const float* alpha,
                                                 float* A, int lda, float* B, int ldb, int max_sweeps);
+/* Number of refinement sweeps the handle's last iclblasStrsmRefined ran. */
+ICLBLAS_API iclblasStatus_t iclblasGetLastRefinementSweeps(iclblasHandle_t handle, int* sweeps);
 
 /* Adjoint of C = alpha*op(A)*op(B) + beta*C for an upstream gradient dC (m x n):
  *   dA (shape of A) = alpha * dC * op(B)^T      (transa == N)

@@ -122,14 +122,17 @@ def test_strsm_config4_plain_and_refined(dev_handle):
     X64 = oracle.dtrsm(0, 1, 0, 0, m, len(cols), 1.0, A, Bs)
     h = dev_handle
     dA = _to_dev(A)
-    res = {}
-    for sweeps in (0, 3):
+    res, ran = {}, {}
+    for sweeps in (0, 1, 3):
         dB = _to_dev(B)
         ib.check(ib.iclblasStrsmRefined(h.value, 0, 1, 0, 0, m, n, 1.0, dA.data_ptr(), m, dB.data_ptr(), m, sweeps))
         torch.cuda.synchronize()
+        ran[sweeps] = h.last_refinement_sweeps()
         X = dB.t().cpu().numpy()[:, cols].astype(np.float64)
         assert np.all(np.isfinite(X))
         res[sweeps] = np.linalg.norm(X - X64) / np.linalg.norm(X64)
     assert res[0] <= 1e-5, res
     assert res[3] <= 2.0 ** -23, res  # FP32 accuracy
     assert res[3] <= res[0]
+    # sweeps stop once a correction no longer changes X at FP32 resolution
+    assert ran[0] == 0 and ran[1] == 1 and 1 <= ran[3] <= 3, ran
