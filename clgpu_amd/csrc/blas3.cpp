@@ -213,10 +213,11 @@ void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n
 namespace {
 
 constexpr int kLeaf = 64;
-// Blocked STRSM above 64 unknowns: the panel solve (256 unknowns per launch
-// plus a GEMM update of the rest) runs when no implementation is forced and
-// kTrsmPanelDefault is set; the launch-per-64-block solve and the single-launch
-// dataflow solve are selectable by name.
+// Blocked STRSM above 64 unknowns: the panel solve with inverted 256 x 256
+// diagonal panels (one GEMM per panel for the solve plus a GEMM update of the
+// rest) runs when no implementation is forced and kTrsmPanelDefault is set; the
+// panel solve with the in-workgroup panel kernel, the launch-per-64-block solve
+// and the single-launch dataflow solve are selectable by name.
 constexpr bool kTrsmPanelDefault = true;
 constexpr int kPanel = 256;
 
@@ -547,16 +548,28 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
         const int nblk = (order + kLeaf - 1) / kLeaf;
         const size_t inv_elems = (size_t)nblk * kLeaf * kLeaf;
         const bool dataflow = h->forced_impl == "strsm_dataflow64";
-        const bool panel = h->forced_impl == "strsm_panel256" ||
+        const std::string& f = h->forced_impl;
+        const bool other_trsm = f == "strsm_panel256" || f == "strsm_blocked_inv64" || f == "strsm_dataflow64";
+        const bool panel_inv = f == "strsm_panel256_inv" || (kTrsmPanelDefault && !other_trsm);
+        const bool panel = panel_inv || h->forced_impl == "strsm_panel256" ||
                            (kTrsmPanelDefault && h->forced_impl != "strsm_blocked_inv64" && !dataflow);
         if (panel) {
             // X in original order with B's shape (m x n, ld m); the canonical X view
-            // mirrors B's. Panels of 256 unknowns are solved in one launch each, and
-            // one FP32 GEMM per panel updates every later unknown, in original
-            // coordinates (positive strides).
+            // mirrors B's. Panels of 256 unknowns are solved one after the other,
+            // and one FP32 GEMM per panel updates every later unknown, in original
+            // coordinates (positive strides). strsm_panel256 solves each panel with
+            // the in-workgroup panel kernel; strsm_panel256_inv first forms the
+            // inverse of every full 256 x 256 diagonal panel at once (the panel
+            // kernel on identity right-hand sides, all panels in one launch: they
+            // do not depend on B) and then solves each panel with one GEMM
+            // X_p = W_p B_p, which spreads over the whole GPU.
+            const int nfull = panel_inv ? order / kPanel : 0;
+            const size_t w_elems = (size_t)nfull * kPanel * kPanel, e_elems = nfull ? (size_t)order * kPanel : 0;
             float* ws = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
-                (inv_elems + (size_t)m * n) * sizeof(float), h->stream));
+                (inv_elems + (size_t)m * n + w_elems + e_elems) * sizeof(float), h->stream));
             float* Xb = ws + inv_elems;
+            float* W = Xb + (size_t)m * n;
+            float* E = W + w_elems;
             a.invD = ws;
             if (t.left) {
                 a.X = t.fwd ? Xb : Xb + e;
@@ -568,13 +581,43 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
                 a.xv_sj = 1;
             }
             hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
+            if (nfull) {
+                // W_p (256 x 256, column-major) such that the panel's unknowns in
+                // original order are X_p = W_p B_p (left) or B_p W_p (right): the
+                // canonical inverse inv_c, written reversed in both indices for a
+                // backward solve and transposed for a right-side solve, through the
+                // X view of a batched panel solve of L_pp Y = I.
+                hip_check(icl_identity_stack(E, order, kPanel, h->stream), "identity_stack");
+                icl_trsm_blocked_args ai = a;
+                ai.nlines = kPanel;
+                ai.B = E;
+                ai.x_si = 1;
+                ai.x_sj = order;
+                const long row_s = t.left ? 1 : kPanel, col_s = t.left ? kPanel : 1; // canonical (row, line) strides
+                const long last = (long)(kPanel - 1) * (row_s + col_s);
+                ai.X = t.fwd ? W : W + last;
+                ai.xv_si = t.fwd ? row_s : -row_s;
+                ai.xv_sj = t.fwd ? col_s : -col_s;
+                ai.x_batch = (long)kPanel * kPanel;
+                hip_check(icl_trsm_panel_batch(&ai, nfull, h->stream), "trsm_panel_batch");
+            }
             for (int ps = 0; ps < order; ps += kPanel) {
                 const int pe = std::min(order, ps + kPanel);
-                hip_check(icl_trsm_panel(&a, ps, pe, h->stream), "trsm_panel");
-                if (pe == order) break;
                 const int plen = pe - ps, rlen = order - pe;
                 const int p_lo = t.fwd ? ps : order - pe; // original index of the panel's first unknown
                 const int r_lo = t.fwd ? pe : 0;          // original index of the first later unknown
+                if (plen == kPanel && ps / kPanel < nfull) {
+                    const float* Wp = W + (size_t)(ps / kPanel) * kPanel * kPanel;
+                    if (t.left)
+                        sgemm_dev(h, ICL_MATH_FP32, 0, 0, kPanel, n, kPanel, 1.0f, Wp, kPanel, B + p_lo, ldb, 0.0f,
+                                  Xb + p_lo, m, ICL_MASK_NONE, false);
+                    else
+                        sgemm_dev(h, ICL_MATH_FP32, 0, 0, m, kPanel, kPanel, 1.0f, B + (long)p_lo * ldb, ldb, Wp, kPanel,
+                                  0.0f, Xb + (long)p_lo * m, m, ICL_MASK_NONE, false);
+                } else {
+                    hip_check(icl_trsm_panel(&a, ps, pe, h->stream), "trsm_panel");
+                }
+                if (pe == order) break;
                 int op = 0;
                 if (t.left) {
                     const float* As = t.sub(r_lo, p_lo, &op);
@@ -587,7 +630,7 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
                 }
             }
             hip_check(icl_axpby_2d(m, n, alpha, Xb, m, 0.0f, B, ldb, h->stream), "axpby");
-            if (record) h->last_impl = "strsm_panel256";
+            if (record) h->last_impl = panel_inv ? "strsm_panel256_inv" : "strsm_panel256";
         } else if (dataflow) {
             // invD, the tile-major X (whole 64 x 64 tiles), then the flags / ticket / abort words
             const int ncb = (a.nlines + kLeaf - 1) / kLeaf;

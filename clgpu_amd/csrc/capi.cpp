@@ -28,11 +28,13 @@ using namespace icl::capi;
 namespace {
 
 // STRSM implementations above 64 unknowns (forceable by name): the panel solve
-// (default: 256 unknowns per launch, the rest updated by a GEMM), the
-// launch-per-64-block solve with inverted diagonal blocks, the single-launch
-// dataflow solve, and the recursive split into diagonal-block solves plus GEMM
-// updates.
-const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_dataflow64", "strsm_panel256", "strsm_recursive"};
+// with inverted 256 x 256 panels (default: each panel one GEMM with its inverse,
+// the rest updated by a GEMM), the panel solve with the in-workgroup panel
+// kernel, the launch-per-64-block solve with inverted diagonal blocks, the
+// single-launch dataflow solve, and the recursive split into diagonal-block
+// solves plus GEMM updates.
+const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_dataflow64", "strsm_panel256", "strsm_panel256_inv",
+                                  "strsm_recursive"};
 
 // Shared GEMM validation; *work is set when the call has work to do.
 iclblasStatus_t gemm_checks(iclblasHandle_t handle, int transa, int transb, int m, int n, int k, const void* alpha,

@@ -166,6 +166,7 @@ typedef struct icl_trsm_blocked_args {
     int unit;
     int* sync;       /* dataflow solve only: icl_trsm_dataflow_sync_ints(...) ints, zeroed by invert_blocks */
     int* error_word; /* dataflow solve only: set to 1 (never cleared by the kernels) when a wait is aborted */
+    long x_batch;    /* icl_trsm_panel_batch only: X offset (elements) between consecutive panels */
 } icl_trsm_blocked_args;
 hipError_t icl_trsm_invert_blocks(const icl_trsm_blocked_args* a, hipStream_t stream);
 hipError_t icl_trsm_step(const icl_trsm_blocked_args* a, int block, hipStream_t stream);
@@ -191,6 +192,13 @@ int icl_trsm_dataflow_sync_ints(int nsys, int nlines);
  * [ps, pe) through the X view. B rows of the panel are read, not written; rows
  * outside the panel are not touched (the caller updates them, e.g. with a GEMM). */
 hipError_t icl_trsm_panel(const icl_trsm_blocked_args* a, int ps, int pe, hipStream_t stream);
+/* The panel solve for panels 0 .. npanels - 1 (rows [256 p, 256 p + 256)) in
+ * one launch, panel p writing its rows of X at a->X + p * a->x_batch, with row
+ * index relative to the panel's first row (x_batch > 0). */
+hipError_t icl_trsm_panel_batch(const icl_trsm_blocked_args* a, int npanels, hipStream_t stream);
+/* E (nsys x width, column-major, ld nsys) = a vertical stack of width x width
+ * identity blocks: E(r, l) = 1 if r % width == l, else 0. */
+hipError_t icl_identity_stack(float* E, int nsys, int width, hipStream_t stream);
 hipError_t icl_trsm_dataflow(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream);
 
 /* R = fp32(alpha * B0 - P * Q) with the products and the sum in FP64.

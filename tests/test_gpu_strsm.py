@@ -53,9 +53,11 @@ def _problem(side, uplo, diag, m, n, seed=41, pad=3):
     return A, B
 
 
-@pytest.mark.parametrize("impl", ["strsm_blocked_inv64", "strsm_dataflow64", "strsm_panel256", "strsm_recursive"])
+@pytest.mark.parametrize("impl", ["strsm_blocked_inv64", "strsm_dataflow64", "strsm_panel256", "strsm_panel256_inv",
+                                  "strsm_recursive"])
 @pytest.mark.parametrize("v", VARIANTS, ids=vid)
-@pytest.mark.parametrize("shape", [(5, 3), (64, 64), (65, 17), (200, 130), (300, 77), (600, 40)],
+@pytest.mark.parametrize("shape", [(5, 3), (64, 64), (65, 17), (200, 130), (300, 77), (600, 40), (70, 520),
+                                   (512, 33)],
                          ids=lambda s: f"{s[0]}x{s[1]}")
 def test_strsm_variants_vs_fp64(h, impl, v, shape):
     side, uplo, trans, diag = v
