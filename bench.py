@@ -149,8 +149,8 @@ def sgemm_rows(h, ib, stream, steps: int) -> dict:
 
 
 def ctrsm_rows(h, ib, stream, steps: int) -> dict:
-    """CTRSM left/lower/{N, C}/non-unit 4096 x 1024 (complex), FP32 path;
-    forward error against a complex FP64 solve computed with torch on the GPU.
+    """CTRSM left/lower/{N, C}/non-unit 4096 x 1024 (complex), FP32 path, the
+    default panel solve (and op N with the recursive solve forced); forward error against a complex FP64 solve computed with torch on the GPU.
     Useful flops 4 m^2 n (one complex multiply-add = 8 real flops, m^2 n / 2 of them)."""
     import torch
 
@@ -165,7 +165,8 @@ def ctrsm_rows(h, ib, stream, steps: int) -> dict:
     tB = torch.from_numpy(np.ascontiguousarray(Bm.T)).to(dev)
     L64 = torch.nan_to_num(tT.to(torch.complex128).t()).tril()
     out = {}
-    for label, trans in (("NN", 0), ("CN", 2)):
+    for label, trans, impl in (("NN", 0, None), ("CN", 2, None), ("NN_recursive", 0, "ctrsm_recursive_inv64")):
+        h.set_implementation(impl)
         opL = L64 if trans == 0 else L64.conj().t()
         X64 = torch.linalg.solve_triangular(opL, tB.to(torch.complex128).t(), upper=trans != 0)
         X = tB.clone()
@@ -179,6 +180,7 @@ def ctrsm_rows(h, ib, stream, steps: int) -> dict:
         out[label] = {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
                       "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA, "forward_rel_err_fro": fe}
         del X, Xw, X64, opL
+    h.set_implementation(None)
     del tT, tB, L64
     torch.cuda.empty_cache()
     return out

@@ -469,21 +469,79 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
         return;
     }
     const int nblk = (t.order + kLeaf - 1) / kLeaf;
-    float* invD = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
-        (size_t)nblk * kLeaf * kLeaf * 2 * sizeof(float), h->stream));
     icl_ctrsm_blocked_args ia{};
     ia.nsys = t.order;
     t.canonical_l(&ia.L, &ia.l_si, &ia.l_sk);
     ia.unit = t.unit ? 1 : 0;
     ia.conj = t.conj ? 1 : 0;
-    ia.invD = invD;
+    const float one[2] = {1.0f, 0.0f}, minus_one[2] = {-1.0f, 0.0f}, zero[2] = {0.0f, 0.0f};
+    if (h->forced_impl == "ctrsm_recursive_inv64") {
+        float* invD = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
+            (size_t)nblk * kLeaf * kLeaf * 2 * sizeof(float), h->stream));
+        ia.invD = invD;
+        hip_check(icl_ctrsm_invert_blocks(&ia, h->stream), "ctrsm_invert_blocks");
+        t.invD = invD;
+        t.solve(0, t.order, one);
+        if (alpha[0] != 1.0f || alpha[1] != 0.0f)
+            hip_check(icl_caxpby_2d(m, n, alpha[0], alpha[1], B, ldb, 0.0f, 0.0f, B, ldb, h->stream), "caxpby");
+        h->last_impl = "ctrsm_recursive_inv64";
+        return;
+    }
+    // Panels of 256 unknowns: the inverse W_p of every diagonal panel is formed
+    // up front (one workgroup per panel, from the inverted 64 x 64 blocks; none
+    // depends on B), then panel by panel X_p = W_p B_p (left) or B_p W_p (right)
+    // with the panel's right-hand sides already updated, and one CGEMM updates
+    // every later unknown. W_p is stored in original coordinates: reversed in both
+    // indices for a backward solve, transposed for a right-side solve
+    // (X op(A) = B is op(A)^T X^T = B^T, so X = B inv(L)^T). X is kept apart from
+    // B (m x n, ld m) and copied back scaled by alpha at the end.
+    const int npanels = (t.order + kPanel - 1) / kPanel;
+    const size_t inv_floats = (size_t)nblk * kLeaf * kLeaf * 2, x_floats = (size_t)m * n * 2;
+    const size_t w_floats = (size_t)npanels * kPanel * kPanel * 2;
+    float* ws = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
+        (inv_floats + x_floats + w_floats) * sizeof(float), h->stream));
+    float* Xb = ws + inv_floats;
+    float* W = Xb + x_floats;
+    ia.invD = ws;
     hip_check(icl_ctrsm_invert_blocks(&ia, h->stream), "ctrsm_invert_blocks");
-    t.invD = invD;
-    const float one[2] = {1.0f, 0.0f};
-    t.solve(0, t.order, one);
-    if (alpha[0] != 1.0f || alpha[1] != 0.0f)
-        hip_check(icl_caxpby_2d(m, n, alpha[0], alpha[1], B, ldb, 0.0f, 0.0f, B, ldb, h->stream), "caxpby");
-    h->last_impl = "ctrsm_recursive_inv64";
+    icl_ctrsm_panel_inv_args pa{};
+    pa.nsys = t.order;
+    pa.L = ia.L;
+    pa.l_si = ia.l_si;
+    pa.l_sk = ia.l_sk;
+    pa.conj = ia.conj;
+    pa.invD = ws;
+    pa.W = W;
+    pa.w_rs = t.left ? 1 : kPanel;
+    pa.w_cs = t.left ? kPanel : 1;
+    pa.rev = t.fwd ? 0 : 1;
+    hip_check(icl_ctrsm_invert_panels(&pa, h->stream), "ctrsm_invert_panels");
+    for (int ps = 0; ps < t.order; ps += kPanel) {
+        const int pe = std::min(t.order, ps + kPanel);
+        const int plen = pe - ps, rlen = t.order - pe;
+        const int p_lo = t.fwd ? ps : t.order - pe; // original index of the panel's first unknown
+        const int r_lo = t.fwd ? pe : 0;            // original index of the first later unknown
+        const float* Wp = W + (size_t)(ps / kPanel) * kPanel * kPanel * 2;
+        if (t.left)
+            cgemm_dev(h, ICL_MATH_FP32, 0, 0, plen, n, plen, one, Wp, kPanel, B + 2 * (long)p_lo, ldb, zero,
+                      Xb + 2 * (long)p_lo, m);
+        else
+            cgemm_dev(h, ICL_MATH_FP32, 0, 0, m, plen, plen, one, B + 2 * (long)p_lo * ldb, ldb, Wp, kPanel, zero,
+                      Xb + 2 * (long)p_lo * m, m);
+        if (pe == t.order) break;
+        int op = 0;
+        if (t.left) {
+            const float* As = t.sub(r_lo, p_lo, &op);
+            cgemm_dev(h, ICL_MATH_FP32, op, 0, rlen, n, plen, minus_one, As, lda, Xb + 2 * (long)p_lo, m, one,
+                      B + 2 * (long)r_lo, ldb);
+        } else {
+            const float* As = t.sub(p_lo, r_lo, &op);
+            cgemm_dev(h, ICL_MATH_FP32, 0, op, m, rlen, plen, minus_one, Xb + 2 * (long)p_lo * m, m, As, lda, one,
+                      B + 2 * (long)r_lo * ldb, ldb);
+        }
+    }
+    hip_check(icl_caxpby_2d(m, n, alpha[0], alpha[1], Xb, m, 0.0f, 0.0f, B, ldb, h->stream), "caxpby");
+    h->last_impl = "ctrsm_panel256_inv";
 }
 
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,

@@ -32,9 +32,11 @@ namespace {
 // the rest updated by a GEMM), the panel solve with the in-workgroup panel
 // kernel, the launch-per-64-block solve with inverted diagonal blocks, the
 // single-launch dataflow solve, and the recursive split into diagonal-block
-// solves plus GEMM updates.
+// solves plus GEMM updates. CTRSM above 64 unknowns: the panel solve with
+// inverted 256 x 256 panels (default) and the recursive split whose leaves are
+// products with the inverted 64 x 64 diagonal blocks.
 const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_dataflow64", "strsm_panel256", "strsm_panel256_inv",
-                                  "strsm_recursive"};
+                                  "strsm_recursive", "ctrsm_panel256_inv", "ctrsm_recursive_inv64"};
 
 // Shared GEMM validation; *work is set when the call has work to do.
 iclblasStatus_t gemm_checks(iclblasHandle_t handle, int transa, int transb, int m, int n, int k, const void* alpha,

@@ -101,17 +101,24 @@ namespace {
 // handles unknowns [64 b, 64 b + nb)): lane j solves L_bb y = e_j with the
 // reference's complex arithmetic and column j of inv(L_bb) is written to
 // invD[b * 4096 + j * 64 + i] (interleaved complex, rows and columns beyond
-// nb zero). L(i, k) = L[i * l_si + k * l_sk], conjugated when conj.
-__global__ void __launch_bounds__(64) ctrsm_invert_kernel(icl_ctrsm_blocked_args a)
+// nb zero). L(i, k) = L[i * l_si + k * l_sk], conjugated when conj. The rows of
+// y are split over four waves (wave w owns rows 16 w .. 16 w + 15); per pivot
+// the owning wave divides and publishes y_i through LDS, then every wave
+// updates its later rows -- the same operations in the same order as a
+// one-wave substitution, a quarter of them per wave.
+constexpr int kInvRows = kNB / 4;
+__global__ void __launch_bounds__(256) ctrsm_invert_kernel(icl_ctrsm_blocked_args a)
 {
     __shared__ CDiagBlock s;
-    const int lane = threadIdx.x;
+    __shared__ float2 piv[2][kNB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = blockIdx.x;
     const int r0 = b * kNB;
     const int nb = min(kNB, a.nsys - r0);
     const float2* L = reinterpret_cast<const float2*>(a.L) + (long)r0 * (a.l_si + a.l_sk);
     const float sg = a.conj ? -1.0f : 1.0f;
-    for (int i = 0; i < kNB; ++i) {
+    for (int ii = 0; ii < kInvRows; ++ii) {
+        const int i = kInvRows * wave + ii;
         float2 v = make_float2(0.0f, 0.0f);
         if (i < nb && lane < nb && lane > i) {
             v = L[(long)lane * a.l_si + (long)i * a.l_sk];
@@ -119,7 +126,7 @@ __global__ void __launch_bounds__(64) ctrsm_invert_kernel(icl_ctrsm_blocked_args
         }
         s.Lt[i][lane] = v;
     }
-    if (lane < nb) {
+    if (wave == 0 && lane < nb) {
         float2 d = make_float2(1.0f, 0.0f);
         if (!a.unit) {
             d = L[(long)lane * (a.l_si + a.l_sk)];
@@ -128,26 +135,34 @@ __global__ void __launch_bounds__(64) ctrsm_invert_kernel(icl_ctrsm_blocked_args
         s.diag[lane] = d;
     }
     __syncthreads();
-    float2 x[kNB];
+    const int row0 = kInvRows * wave;
+    float2 x[kInvRows];
 #pragma unroll
-    for (int i = 0; i < kNB; ++i) x[i] = make_float2(i == lane && lane < nb ? 1.0f : 0.0f, 0.0f);
+    for (int r = 0; r < kInvRows; ++r) x[r] = make_float2(row0 + r == lane && lane < nb ? 1.0f : 0.0f, 0.0f);
 #pragma unroll
     for (int i = 0; i < kNB; ++i) {
         if (i < nb) {
-            float2 xi = x[i];
-            if (!a.unit) xi = c_div(xi, s.diag[i]);
-            x[i] = xi;
+            if (wave == i / kInvRows) {
+                float2 xi = x[i % kInvRows];
+                if (!a.unit) xi = c_div(xi, s.diag[i]);
+                x[i % kInvRows] = xi;
+                piv[i & 1][lane] = xi;
+            }
+            __syncthreads(); // piv[i & 1] is rewritten two pivots later, after the next barrier
+            const float2 xi = piv[i & 1][lane];
 #pragma unroll
-            for (int k = i + 1; k < kNB; ++k) {
-                const float2 t = c_mul(s.Lt[i][k], xi);
-                x[k].x -= t.x;
-                x[k].y -= t.y;
+            for (int r = 0; r < kInvRows; ++r) {
+                if (row0 + r > i) {
+                    const float2 t = c_mul(s.Lt[i][row0 + r], xi);
+                    x[r].x -= t.x;
+                    x[r].y -= t.y;
+                }
             }
         }
     }
-    float2* col = reinterpret_cast<float2*>(a.invD) + (long)b * kNB * kNB + (long)lane * kNB;
+    float2* col = reinterpret_cast<float2*>(a.invD) + (long)b * kNB * kNB + (long)lane * kNB + row0;
 #pragma unroll
-    for (int i = 0; i < kNB; ++i) col[i] = (lane < nb && i < nb) ? x[i] : make_float2(0.0f, 0.0f);
+    for (int r = 0; r < kInvRows; ++r) col[r] = (lane < nb && row0 + r < nb) ? x[r] : make_float2(0.0f, 0.0f);
 }
 
 // X_b = s_post * inv(L_bb) B_b for one diagonal block and all lines, in place:
@@ -243,13 +258,155 @@ __global__ void __launch_bounds__(kApThreads) ctrsm_apply_kernel(icl_ctrsm_apply
     }
 }
 
+// Inverses of the 256 x 256 diagonal panels (see icl_ctrsm_invert_panels).
+// Workgroup (p, j) forms the 64-wide column block j of panel p's inverse: the
+// column blocks of a triangular inverse do not depend on each other
+// (M_ij = -inv(L_ii) sum_{k=j}^{i-1} L_ik M_kj reads only column block j), so
+// the four run side by side. Four waves per workgroup; each 64 x 64 complex
+// product runs on v_mfma_f32_32x32x2_f32 with the operands staged in LDS as
+// planar re / im tiles, A as [k][row] and B as [k][col], each wave owning a
+// 32 x 32 quadrant. Computed blocks go to W and are read back from there by the
+// later products; a fence at agent scope around the barrier makes them visible
+// across the workgroup.
+constexpr int kPanRows = 256;
+constexpr int kPnLd = kNB + 1;
+constexpr size_t kPanInvLds = sizeof(float) * 4 * kNB * kPnLd;
+
+__device__ __forceinline__ void publish()
+{
+    __threadfence();
+    __syncthreads();
+    __threadfence();
+}
+
+__global__ void __launch_bounds__(256) ctrsm_invert_panels_kernel(icl_ctrsm_panel_inv_args a)
+{
+    extern __shared__ float sm[];
+    float(*Ar)[kPnLd] = reinterpret_cast<float(*)[kPnLd]>(sm); // A[k][row]
+    float(*Ai)[kPnLd] = Ar + kNB;
+    float(*Br)[kPnLd] = Ai + kNB; // B[k][col]
+    float(*Bi)[kPnLd] = Br + kNB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
+    const int pnl = blockIdx.x, j = blockIdx.y, ps = pnl * kPanRows;
+    const int plen = min(kPanRows, a.nsys - ps), nb = (plen + kNB - 1) / kNB;
+    if (j >= nb) return; // the whole workgroup leaves before any barrier
+    const float2* L = reinterpret_cast<const float2*>(a.L);
+    const float2* invD = reinterpret_cast<const float2*>(a.invD);
+    float2* W = reinterpret_cast<float2*>(a.W) + (long)pnl * kPanRows * kPanRows;
+    const float sg = a.conj ? -1.0f : 1.0f;
+    auto w_at = [&](int r, int c) -> long {
+        return a.rev ? (long)(plen - 1 - r) * a.w_rs + (long)(plen - 1 - c) * a.w_cs : (long)r * a.w_rs + (long)c * a.w_cs;
+    };
+    constexpr int PER = kNB * kNB / 256; // elements of a 64 x 64 block per thread; e = tid + 256 q: (row e % 64, col e / 64)
+
+    // Blocks of column block j above the diagonal are 0, the diagonal one is inv(L_jj).
+    for (int bi = 0; bi <= j; ++bi) {
+        float2 v[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + 256 * q, r = e % kNB, c = e / kNB;
+            v[q] = bi == j ? invD[((long)(ps / kNB + j) * kNB + c) * kNB + r] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + 256 * q, r = kNB * bi + e % kNB, c = kNB * j + e / kNB;
+            if (r < plen && c < plen) W[w_at(r, c)] = v[q];
+        }
+    }
+    publish();
+
+    // acc (+)= A B for the staged tiles; lanes along rows of the result.
+    auto mma = [&](f32x16& accr, f32x16& acci) {
+#pragma unroll 8
+        for (int kk = 0; kk < kNB; kk += 2) {
+            const int kr = kk + (lane >> 5);
+            const float ar = Ar[kr][wr + (lane & 31)], ai = Ai[kr][wr + (lane & 31)];
+            const float br = Br[kr][wc + (lane & 31)], bi = Bi[kr][wc + (lane & 31)];
+            accr = icl::mfma_32x32x2_f32(br, ar, accr);
+            acci = icl::mfma_32x32x2_f32(bi, ar, acci);
+            accr = icl::mfma_32x32x2_f32(bi, -ai, accr);
+            acci = icl::mfma_32x32x2_f32(br, ai, acci);
+        }
+    };
+    for (int i = j + 1; i < nb; ++i) {
+        // S = sum_{k=j}^{i-1} L_ik M_kj
+        f32x16 sr = f32x16{}, si = f32x16{};
+        for (int k = j; k < i; ++k) {
+            float2 va[PER], vb[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = tid + 256 * q, x = e % kNB, y = e / kNB; // A: (row x, k y); B: (k x, col y)
+                const int lr = ps + kNB * i + x, lk = ps + kNB * k + y;
+                float2 l = make_float2(0.0f, 0.0f);
+                if (lr < a.nsys && lk < a.nsys) {
+                    l = L[(long)lr * a.l_si + (long)lk * a.l_sk];
+                    l.y *= sg;
+                }
+                va[q] = l;
+                const int mr = kNB * k + x, mc = kNB * j + y;
+                vb[q] = (mr < plen && mc < plen) ? W[w_at(mr, mc)] : make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = tid + 256 * q, x = e % kNB, y = e / kNB;
+                Ar[y][x] = va[q].x;
+                Ai[y][x] = va[q].y;
+                Br[x][y] = vb[q].x;
+                Bi[x][y] = vb[q].y;
+            }
+            __syncthreads();
+            mma(sr, si);
+            __syncthreads();
+        }
+        // M_ij = -inv(L_ii) S: S becomes the B tile, inv(L_ii) the A tile.
+        float2 vd[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + 256 * q, x = e % kNB, y = e / kNB; // (row x, k y)
+            vd[q] = invD[((long)(ps / kNB + i) * kNB + y) * kNB + x];
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            Br[wr + (lane & 31)][wc + icl::mfma32_row(lane, r)] = sr[r];
+            Bi[wr + (lane & 31)][wc + icl::mfma32_row(lane, r)] = si[r];
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + 256 * q, x = e % kNB, y = e / kNB;
+            Ar[y][x] = vd[q].x;
+            Ai[y][x] = vd[q].y;
+        }
+        __syncthreads();
+        f32x16 mr = f32x16{}, mi = f32x16{};
+        mma(mr, mi);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = kNB * i + wr + (lane & 31), col = kNB * j + wc + icl::mfma32_row(lane, r);
+            if (row < plen && col < plen) W[w_at(row, col)] = make_float2(-mr[r], -mi[r]);
+        }
+        publish();
+    }
+}
+
 } // namespace
+
+extern "C" hipError_t icl_ctrsm_invert_panels(const icl_ctrsm_panel_inv_args* a, hipStream_t stream)
+{
+    if (a->nsys <= 0) return hipSuccess;
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(ctrsm_invert_panels_kernel),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPanInvLds);
+    if (attr != hipSuccess) return attr;
+    const unsigned panels = (unsigned)((a->nsys + kPanRows - 1) / kPanRows);
+    hipLaunchKernelGGL(ctrsm_invert_panels_kernel, dim3(panels, kPanRows / kNB), dim3(256), kPanInvLds, stream, *a);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t icl_ctrsm_invert_blocks(const icl_ctrsm_blocked_args* a, hipStream_t stream)
 {
     if (a->nsys <= 0) return hipSuccess;
     const unsigned blocks = (unsigned)((a->nsys + kNB - 1) / kNB);
-    hipLaunchKernelGGL(ctrsm_invert_kernel, dim3(blocks), dim3(64), 0, stream, *a);
+    hipLaunchKernelGGL(ctrsm_invert_kernel, dim3(blocks), dim3(256), 0, stream, *a);
     return hipGetLastError();
 }
 

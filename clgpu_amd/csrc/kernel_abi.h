@@ -133,6 +133,27 @@ typedef struct icl_ctrsm_blocked_args {
 } icl_ctrsm_blocked_args;
 hipError_t icl_ctrsm_invert_blocks(const icl_ctrsm_blocked_args* a, hipStream_t stream);
 
+/* Inverses of the 256 x 256 diagonal panels of the canonical complex system
+ * (panel p = unknowns [256 p, 256 p + plen), plen = min(256, nsys - 256 p)),
+ * one workgroup per 64-wide column block of a panel, from the inverted 64 x 64
+ * blocks invD
+ * (icl_ctrsm_invert_blocks): M_jj = inv(L_jj) and, for i > j,
+ * M_ij = -inv(L_ii) * sum_{k=j}^{i-1} L_ik M_kj. Element (r, c), r, c < plen, of
+ * panel p's inverse is stored (interleaved complex) at element
+ * p * 65536 + (rev ? (plen-1-r) * w_rs + (plen-1-c) * w_cs : r * w_rs + c * w_cs)
+ * of W; entries above the block diagonal are stored as 0. */
+typedef struct icl_ctrsm_panel_inv_args {
+    int nsys;
+    const float* L;
+    long l_si, l_sk;
+    int conj;
+    const float* invD;
+    float* W;
+    long w_rs, w_cs;
+    int rev;
+} icl_ctrsm_panel_inv_args;
+hipError_t icl_ctrsm_invert_panels(const icl_ctrsm_panel_inv_args* a, hipStream_t stream);
+
 /* X_b = s_post * inv(L_bb) X_b in place for one diagonal block of nb <= 64
  * unknowns and nlines lines: X(i, j) = X[i * x_si + j * x_sj] (interleaved
  * complex), invD = that block's dense inverse. */

@@ -1,9 +1,10 @@
 """CTRSM parity on the GPU through the C ABI: the reference's known answers
 (tests/iclBLAS/test_Ctrsm.cpp, EXPECT_COMPLEX_NEAR with 4e-4), all 24
 side/uplo/trans(N, T, C)/diag variants against the complex FP64 substitution of
-the oracle at sizes that take one diagonal block and several, never-read
-triangles and unit diagonals poisoned with NaN, padded B, alpha == 0, and the
-FP32 path at 1024 x 512."""
+the oracle at sizes that take one diagonal block, several, and several 256-unknown
+panels with a partial last one (the default panel solve and the recursive one),
+never-read triangles and unit diagonals poisoned with NaN, padded B,
+alpha == 0, and the FP32 path at 1024 x 512."""
 import json
 from pathlib import Path
 
@@ -54,11 +55,14 @@ def _problem(side, uplo, diag, m, n, seed=41, pad=3):
     return A, B
 
 
+@pytest.mark.parametrize("impl", [None, "ctrsm_recursive_inv64"], ids=["default", "recursive"])
 @pytest.mark.parametrize("v", VARIANTS, ids=vid)
-@pytest.mark.parametrize("shape", [(5, 3), (64, 64), (65, 17), (200, 130)], ids=lambda s: f"{s[0]}x{s[1]}")
-def test_ctrsm_variants_vs_fp64(h, v, shape):
+@pytest.mark.parametrize("shape", [(5, 3), (64, 64), (65, 17), (200, 130), (600, 40), (70, 520)],
+                         ids=lambda s: f"{s[0]}x{s[1]}")
+def test_ctrsm_variants_vs_fp64(h, v, shape, impl):
     side, uplo, trans, diag = v
     m, n = shape
+    h.set_implementation(impl)
     A, B = _problem(side, uplo, diag, m, n)
     alpha = -0.75 + 0.4j
     X64 = oracle.ztrsm(side, uplo, trans, diag, m, n, alpha, A, B)
@@ -74,7 +78,7 @@ def test_ctrsm_variants_vs_fp64(h, v, shape):
     order = m if side == 0 else n
     # within a small multiple of the reference's own float error, or of order*u
     assert err_gpu <= max(8 * err_cpu, 8 * order * 2.0 ** -24), (err_gpu, err_cpu)
-    assert h.last_implementation() == ("ctrsm_diag64" if order <= 64 else "ctrsm_recursive_inv64")
+    assert h.last_implementation() == ("ctrsm_diag64" if order <= 64 else impl or "ctrsm_panel256_inv")
 
 
 @pytest.mark.parametrize("v", VARIANTS[::5], ids=vid)
