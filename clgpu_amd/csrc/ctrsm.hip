@@ -117,14 +117,21 @@ __global__ void __launch_bounds__(256) ctrsm_invert_kernel(icl_ctrsm_blocked_arg
     const int nb = min(kNB, a.nsys - r0);
     const float2* L = reinterpret_cast<const float2*>(a.L) + (long)r0 * (a.l_si + a.l_sk);
     const float sg = a.conj ? -1.0f : 1.0f;
+    // All loads first (an entry outside the strict lower triangle reads the
+    // block's first diagonal element instead and is replaced by 0), so that they
+    // are in flight together.
+    float2 lv[kInvRows];
+#pragma unroll
     for (int ii = 0; ii < kInvRows; ++ii) {
         const int i = kInvRows * wave + ii;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (i < nb && lane < nb && lane > i) {
-            v = L[(long)lane * a.l_si + (long)i * a.l_sk];
-            v.y *= sg;
-        }
-        s.Lt[i][lane] = v;
+        const bool used = i < nb && lane < nb && lane > i;
+        lv[ii] = L[used ? (long)lane * a.l_si + (long)i * a.l_sk : 0];
+    }
+#pragma unroll
+    for (int ii = 0; ii < kInvRows; ++ii) {
+        const int i = kInvRows * wave + ii;
+        const bool used = i < nb && lane < nb && lane > i;
+        s.Lt[i][lane] = used ? make_float2(lv[ii].x, sg * lv[ii].y) : make_float2(0.0f, 0.0f);
     }
     if (wave == 0 && lane < nb) {
         float2 d = make_float2(1.0f, 0.0f);
@@ -135,24 +142,34 @@ __global__ void __launch_bounds__(256) ctrsm_invert_kernel(icl_ctrsm_blocked_arg
         s.diag[lane] = d;
     }
     __syncthreads();
-    const int row0 = kInvRows * wave;
+    const int wv = __builtin_amdgcn_readfirstlane(wave); // the same in every lane: scalar branches below
+    const int row0 = kInvRows * wv;
     float2 x[kInvRows];
 #pragma unroll
     for (int r = 0; r < kInvRows; ++r) x[r] = make_float2(row0 + r == lane && lane < nb ? 1.0f : 0.0f, 0.0f);
+    for (int ib = 0; ib < kNB; ib += kInvRows) {
 #pragma unroll
-    for (int i = 0; i < kNB; ++i) {
-        if (i < nb) {
-            if (wave == i / kInvRows) {
-                float2 xi = x[i % kInvRows];
+        for (int ii = 0; ii < kInvRows; ++ii) {
+            const int i = ib + ii;
+            if (i >= nb) break;
+            if (wv == ib / kInvRows) {
+                float2 xi = x[ii];
                 if (!a.unit) xi = c_div(xi, s.diag[i]);
-                x[i % kInvRows] = xi;
+                x[ii] = xi;
                 piv[i & 1][lane] = xi;
             }
             __syncthreads(); // piv[i & 1] is rewritten two pivots later, after the next barrier
             const float2 xi = piv[i & 1][lane];
+            if (wv > ib / kInvRows) {
 #pragma unroll
-            for (int r = 0; r < kInvRows; ++r) {
-                if (row0 + r > i) {
+                for (int r = 0; r < kInvRows; ++r) {
+                    const float2 t = c_mul(s.Lt[i][row0 + r], xi);
+                    x[r].x -= t.x;
+                    x[r].y -= t.y;
+                }
+            } else if (wv == ib / kInvRows) {
+#pragma unroll
+                for (int r = ii + 1; r < kInvRows; ++r) {
                     const float2 t = c_mul(s.Lt[i][row0 + r], xi);
                     x[r].x -= t.x;
                     x[r].y -= t.y;
@@ -333,27 +350,27 @@ __global__ void __launch_bounds__(256) ctrsm_invert_panels_kernel(icl_ctrsm_pane
         // S = sum_{k=j}^{i-1} L_ik M_kj
         f32x16 sr = f32x16{}, si = f32x16{};
         for (int k = j; k < i; ++k) {
+            // All loads first, unconditionally (an entry outside the system reads
+            // element 0 of L or of the panel's W instead and is replaced by 0),
+            // so that they are in flight together.
             float2 va[PER], vb[PER];
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 const int e = tid + 256 * q, x = e % kNB, y = e / kNB; // A: (row x, k y); B: (k x, col y)
                 const int lr = ps + kNB * i + x, lk = ps + kNB * k + y;
-                float2 l = make_float2(0.0f, 0.0f);
-                if (lr < a.nsys && lk < a.nsys) {
-                    l = L[(long)lr * a.l_si + (long)lk * a.l_sk];
-                    l.y *= sg;
-                }
-                va[q] = l;
+                va[q] = L[(lr < a.nsys && lk < a.nsys) ? (long)lr * a.l_si + (long)lk * a.l_sk : 0];
                 const int mr = kNB * k + x, mc = kNB * j + y;
-                vb[q] = (mr < plen && mc < plen) ? W[w_at(mr, mc)] : make_float2(0.0f, 0.0f);
+                vb[q] = W[(mr < plen && mc < plen) ? w_at(mr, mc) : 0];
             }
 #pragma unroll
             for (int q = 0; q < PER; ++q) {
                 const int e = tid + 256 * q, x = e % kNB, y = e / kNB;
-                Ar[y][x] = va[q].x;
-                Ai[y][x] = va[q].y;
-                Br[x][y] = vb[q].x;
-                Bi[x][y] = vb[q].y;
+                const bool lin = ps + kNB * i + x < a.nsys && ps + kNB * k + y < a.nsys;
+                const bool win = kNB * k + x < plen && kNB * j + y < plen;
+                Ar[y][x] = lin ? va[q].x : 0.0f;
+                Ai[y][x] = lin ? sg * va[q].y : 0.0f;
+                Br[x][y] = win ? vb[q].x : 0.0f;
+                Bi[x][y] = win ? vb[q].y : 0.0f;
             }
             __syncthreads();
             mma(sr, si);
