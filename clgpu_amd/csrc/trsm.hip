@@ -102,16 +102,19 @@ __global__ void __launch_bounds__(64) trsm_invert_kernel(icl_trsm_blocked_args a
 
 // ---------------------------------------------------------------------------
 // FP64 residual R = fp32(alpha * B0 - P * Q) on v_mfma_f64_16x16x4_f64.
-// 64 x 64 output tile per workgroup (4 waves, each 32 x 32 = 2 x 2 MFMA blocks),
-// K slabs of 32 staged in LDS as FP64; slabs outside the triangle are skipped
-// and only slabs that cross the diagonal apply the triangle rule. The next slab
-// is read into registers (straight-line, strength-reduced addressing) while the
-// current one is multiplied. Tiles are dispatched longest first (their K range
-// depends on their position relative to the triangle), so the short ones fill
-// the tail.
+// 64 x 64 output tiles, 4 waves per workgroup, each 32 x 32 = 2 x 2 MFMA blocks.
+// K slabs of 32 are staged in LDS as FP32 (converted to FP64 at the fragment
+// read) in two buffers: the next slab is read into registers while the current
+// one is multiplied and stored to the other buffer afterwards, one barrier per
+// slab. Slabs outside the triangle are skipped and only slabs that cross the
+// diagonal apply the triangle rule. The K range of a tile depends on its
+// position along the triangular operand's free dimension, so a workgroup takes
+// tiles t and T - 1 - t there (T tiles): every workgroup then does the same
+// total K, and no CU is left with the long tiles while others idle.
 // ---------------------------------------------------------------------------
 constexpr int RT = 64, RK = 32;
 constexpr int kResPer = RT * RK / 256; // elements of each operand slab per thread
+constexpr int kResLd = RT + 16;        // the four 16-lane row groups of a fragment read hit disjoint banks
 
 // One thread's share of a slab of an operand whose element (i, k) (i < RT along
 // the tile, k < RK along K) sits at X[i * s_i + k * s_k]. KFAST: k is the unit
@@ -131,119 +134,145 @@ struct ResSlab {
 template <bool PT, bool QT>
 __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
 {
-    __shared__ double Ps[RK][RT + 1]; // Ps[k][r] = P(m0 + r, kb + k)
-    __shared__ double Qs[RK][RT + 1]; // Qs[k][c] = Q(kb + k, n0 + c)
+    __shared__ float Ps[2][RK][kResLd]; // Ps[b][k][r] = P(m0 + r, kb + k)
+    __shared__ float Qs[2][RK][kResLd]; // Qs[b][k][c] = Q(kb + k, n0 + c)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_m = (a.M + RT - 1) / RT, tiles_n = (a.N + RT - 1) / RT;
     const int bid = blockIdx.x;
-    int tm, tn;
-    if (a.tri_on == 1) { // K range grows with m (lower) or shrinks (upper)
-        tn = bid % tiles_n;
-        tm = a.tri_lower ? tiles_m - 1 - bid / tiles_n : bid / tiles_n;
-    } else if (a.tri_on == 2) { // K range shrinks with n (lower) or grows (upper)
-        tm = bid % tiles_m;
-        tn = a.tri_lower ? bid / tiles_m : tiles_n - 1 - bid / tiles_m;
+    // This workgroup's tiles: (tm[t], tn[t]) for t < ntiles.
+    int tm[2], tn[2], ntiles = 1;
+    if (a.tri_on == 1) { // the K range depends on the tile row
+        tn[0] = tn[1] = bid % tiles_n;
+        tm[0] = bid / tiles_n;
+        tm[1] = tiles_m - 1 - tm[0];
+        ntiles = tm[1] > tm[0] ? 2 : 1;
+    } else if (a.tri_on == 2) { // on the tile column
+        tm[0] = tm[1] = bid % tiles_m;
+        tn[0] = bid / tiles_m;
+        tn[1] = tiles_n - 1 - tn[0];
+        ntiles = tn[1] > tn[0] ? 2 : 1;
     } else {
-        tm = bid % tiles_m;
-        tn = bid / tiles_m;
+        tm[0] = tm[1] = bid % tiles_m;
+        tn[0] = tn[1] = bid / tiles_m;
     }
-    const int m0 = tm * RT, n0 = tn * RT;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-
-    // K range that can be non-zero for this tile.
-    int k_lo = 0, k_hi = a.K;
-    if (a.tri_on == 1) { // P = op(A) is M x K triangular; rows m0..m0+63
-        if (a.tri_lower) k_hi = min(a.K, m0 + RT);
-        else k_lo = m0;
-    } else if (a.tri_on == 2) { // Q = op(A) is K x N triangular; cols n0..n0+63
-        if (a.tri_lower) k_lo = n0;
-        else k_hi = min(a.K, n0 + RT);
-    }
-    k_lo = (k_lo / RK) * RK;
-
-    f64x4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-
     // P(i, k) = P[i * ldp + k] when PT, P[i + k * ldp] otherwise; likewise Q(k, c)
     // = Q[k * ldq + c] when QT, Q[c * ldq + k] otherwise (c plays the role of i).
     const ResSlab<PT> ps(tid);
     const ResSlab<!QT> qs(tid);
     const long p_si = PT ? a.ldp : 1, p_sk = PT ? 1 : a.ldp;
     const long q_si = QT ? 1 : a.ldq, q_sk = QT ? a.ldq : 1;
-    const int pi = m0 + ps.i0, qi = n0 + qs.i0;
-    float pv[kResPer], qv[kResPer];
-    auto fetch = [&](int kb) {
-        // triangle rule only where the slab crosses the diagonal of the triangular operand
-        const bool p_diag = a.tri_on == 1 && kb < m0 + RT && m0 < kb + RK;
-        const bool q_diag = a.tri_on == 2 && kb < n0 + RT && n0 < kb + RK;
-        const float* pp = a.P + (long)pi * p_si + (long)(kb + ps.k0) * p_sk;
-        const float* qp = a.Q + (long)qi * q_si + (long)(kb + qs.k0) * q_sk;
-#pragma unroll
-        for (int s = 0; s < kResPer; ++s) {
-            const int i = pi + s * ps.kDi, k = kb + ps.k0 + s * ps.kDk;
-            const bool ok = i < a.M && k < a.K;
-            const float x = *(ok ? pp + (long)s * (ps.kDi * p_si + ps.kDk * p_sk) : a.P);
-            float v = ok ? x : 0.0f;
-            if (p_diag) {
-                v = (a.tri_lower ? k > i : k < i) ? 0.0f : v;
-                v = (a.tri_unit && k == i) ? 1.0f : v;
-            }
-            pv[s] = v;
-        }
-#pragma unroll
-        for (int s = 0; s < kResPer; ++s) {
-            const int c = qi + s * qs.kDi, k = kb + qs.k0 + s * qs.kDk;
-            const bool ok = c < a.N && k < a.K;
-            const float x = *(ok ? qp + (long)s * (qs.kDi * q_si + qs.kDk * q_sk) : a.Q);
-            float v = ok ? x : 0.0f;
-            if (q_diag) { // Q(k, c): row k, column c of the triangle
-                v = (a.tri_lower ? c > k : c < k) ? 0.0f : v;
-                v = (a.tri_unit && k == c) ? 1.0f : v;
-            }
-            qv[s] = v;
-        }
-    };
-    if (k_lo < k_hi) fetch(k_lo);
-    for (int kb = k_lo; kb < k_hi; kb += RK) {
-#pragma unroll
-        for (int s = 0; s < kResPer; ++s) {
-            Ps[ps.k0 + s * ps.kDk][ps.i0 + s * ps.kDi] = (double)pv[s];
-            Qs[qs.k0 + s * qs.kDk][qs.i0 + s * qs.kDi] = (double)qv[s];
-        }
-        __syncthreads();
-        if (kb + RK < k_hi) fetch(kb + RK); // in flight during the products below
-#pragma unroll
-        for (int kk = 0; kk < RK; kk += 4) {
-            const int kr = kk + (lane >> 4);
-            double p[2], q[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) p[i] = Ps[kr][wm + 16 * i + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) q[j] = Qs[kr][wn + 16 * j + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = icl::mfma_16x16x4_f64(q[j], p[i], acc[i][j]);
-        }
-        __syncthreads();
-    }
 
+    for (int t = 0; t < ntiles; ++t) {
+        const int m0 = tm[t] * RT, n0 = tn[t] * RT;
+        // K range that can be non-zero for this tile.
+        int k_lo = 0, k_hi = a.K;
+        if (a.tri_on == 1) { // P = op(A) is M x K triangular; rows m0..m0+63
+            if (a.tri_lower) k_hi = min(a.K, m0 + RT);
+            else k_lo = m0;
+        } else if (a.tri_on == 2) { // Q = op(A) is K x N triangular; cols n0..n0+63
+            if (a.tri_lower) k_lo = n0;
+            else k_hi = min(a.K, n0 + RT);
+        }
+        k_lo = (k_lo / RK) * RK;
+
+        f64x4 acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+        const int pi = m0 + ps.i0, qi = n0 + qs.i0;
+        // fetch() only loads (an element outside the matrices reads element 0
+        // instead), so that all loads of a slab are in flight together; stash()
+        // applies the bounds and the triangle rule when it stores to LDS.
+        float pv[kResPer], qv[kResPer];
+        auto fetch = [&](int kb) {
+            const float* pp = a.P + (long)pi * p_si + (long)(kb + ps.k0) * p_sk;
+            const float* qp = a.Q + (long)qi * q_si + (long)(kb + qs.k0) * q_sk;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm + 16 * i + (lane & 15);
-                const int n = n0 + wn + 16 * j + 4 * r + (lane >> 4);
-                if (m < a.M && n < a.N) {
-                    const double b = (double)a.B0[(long)n * a.ldb0 + m];
-                    a.R[(long)n * a.ldr + m] = (float)(a.alpha * b - acc[i][j][r]);
-                }
+            for (int s = 0; s < kResPer; ++s) {
+                const int i = pi + s * ps.kDi, k = kb + ps.k0 + s * ps.kDk;
+                pv[s] = *((i < a.M && k < a.K) ? pp + (long)s * (ps.kDi * p_si + ps.kDk * p_sk) : a.P);
             }
+#pragma unroll
+            for (int s = 0; s < kResPer; ++s) {
+                const int c = qi + s * qs.kDi, k = kb + qs.k0 + s * qs.kDk;
+                qv[s] = *((c < a.N && k < a.K) ? qp + (long)s * (qs.kDi * q_si + qs.kDk * q_sk) : a.Q);
+            }
+        };
+        auto stash = [&](int b, int kb) {
+            // triangle rule only where the slab crosses the diagonal of the triangular operand
+            const bool p_diag = a.tri_on == 1 && kb < m0 + RT && m0 < kb + RK;
+            const bool q_diag = a.tri_on == 2 && kb < n0 + RT && n0 < kb + RK;
+#pragma unroll
+            for (int s = 0; s < kResPer; ++s) {
+                const int i = pi + s * ps.kDi, k = kb + ps.k0 + s * ps.kDk;
+                float v = (i < a.M && k < a.K) ? pv[s] : 0.0f;
+                v = (p_diag && (a.tri_lower ? k > i : k < i)) ? 0.0f : v;
+                v = (p_diag && a.tri_unit && k == i) ? 1.0f : v;
+                Ps[b][ps.k0 + s * ps.kDk][ps.i0 + s * ps.kDi] = v;
+            }
+#pragma unroll
+            for (int s = 0; s < kResPer; ++s) { // Q(k, c): row k, column c of the triangle
+                const int c = qi + s * qs.kDi, k = kb + qs.k0 + s * qs.kDk;
+                float v = (c < a.N && k < a.K) ? qv[s] : 0.0f;
+                v = (q_diag && (a.tri_lower ? c > k : c < k)) ? 0.0f : v;
+                v = (q_diag && a.tri_unit && k == c) ? 1.0f : v;
+                Qs[b][qs.k0 + s * qs.kDk][qs.i0 + s * qs.kDi] = v;
+            }
+        };
+        if (k_lo < k_hi) {
+            fetch(k_lo);
+            stash(0, k_lo);
+            __syncthreads();
+        }
+        int cur = 0;
+        for (int kb = k_lo; kb < k_hi; kb += RK) {
+            const bool more = kb + RK < k_hi;
+            if (more) fetch(kb + RK); // in flight during the products below
+#pragma unroll
+            for (int kk = 0; kk < RK; kk += 4) {
+                const int kr = kk + (lane >> 4);
+                double p[2], q[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) p[i] = (double)Ps[cur][kr][wm + 16 * i + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) q[j] = (double)Qs[cur][kr][wn + 16 * j + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = icl::mfma_16x16x4_f64(q[j], p[i], acc[i][j]);
+            }
+            // The other buffer was last read before the previous barrier.
+            if (more) stash(cur ^ 1, kb + RK);
+            __syncthreads();
+            cur ^= 1;
+        }
+
+        // B0 for the whole fragment first (outside the matrix: element 0), then the stores.
+        const int m_base = m0 + wm + (lane & 15), n_base = n0 + wn + (lane >> 4);
+        float b0[2][2][4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m_base + 16 * i, n = n_base + 16 * j + 4 * r;
+                    b0[i][j][r] = a.B0[(m < a.M && n < a.N) ? (long)n * a.ldb0 + m : 0];
+                }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m_base + 16 * i, n = n_base + 16 * j + 4 * r;
+                    if (m < a.M && n < a.N)
+                        a.R[(long)n * a.ldr + m] = (float)(a.alpha * (double)b0[i][j][r] - acc[i][j][r]);
+                }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -960,7 +989,11 @@ extern "C" hipError_t icl_trsm_scatter(const icl_trsm_blocked_args* a, float alp
 extern "C" hipError_t icl_residual_f64(const icl_residual_args* a, hipStream_t stream)
 {
     if (a->M <= 0 || a->N <= 0) return hipSuccess;
-    const long tiles = (long)((a->M + RT - 1) / RT) * ((a->N + RT - 1) / RT);
+    const long tiles_m = (a->M + RT - 1) / RT, tiles_n = (a->N + RT - 1) / RT;
+    // one workgroup per pair of tiles along the triangular operand's free dimension
+    const long tiles = a->tri_on == 1 ? (tiles_m + 1) / 2 * tiles_n
+                     : a->tri_on == 2 ? tiles_m * ((tiles_n + 1) / 2)
+                                      : tiles_m * tiles_n;
     using Kern = void (*)(icl_residual_args);
     static const Kern kernels[4] = {residual_f64_kernel<false, false>, residual_f64_kernel<false, true>,
                                     residual_f64_kernel<true, false>, residual_f64_kernel<true, true>};

@@ -95,9 +95,10 @@ def test_strsm_alpha_zero_sets_zero_without_reading(h, v):
 
 
 @pytest.mark.parametrize("v", VARIANTS, ids=vid)
-def test_strsm_refined_variants(h, v):
+@pytest.mark.parametrize("shape", [(190, 45), (45, 190), (300, 130)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_strsm_refined_variants(h, v, shape):
     side, uplo, trans, diag = v
-    m, n = 190, 45
+    m, n = shape
     A, B = _problem(side, uplo, diag, m, n, seed=43)
     alpha = 1.25
     X64 = oracle.dtrsm(side, uplo, trans, diag, m, n, alpha, A, B)
