@@ -50,6 +50,12 @@ bool gemm_needs_edge(const icl_gemm_variant& v, const GemmProblem& p)
 // (256*occ), so grids of fewer tiles than CUs favour smaller tiles. Variants
 // with a pack prepass add its HBM time ((M + N) * K * (4 read + 4 written)
 // bytes at 5 TB/s). The bounds-checked instantiation is taken to run 10% slower.
+// For K <= 512 a tile's fixed cost (pipeline fill, C read and write) is no
+// longer small against its K loop: it is counted as kTileOverheadK * (elements
+// of C per wave) extra steps of K, normalised so that the rate at K = 8192 is
+// still the measured one (fitted to 64x64 and 128x128 tiles at K = 256, M up
+// to 3840, N = 1024, where the larger tile took 1.5x as long as the smaller).
+constexpr double kTileOverheadK = 0.12;
 float gemm_score(const icl_gemm_variant& v, const GemmProblem& p)
 {
     const icl_gemm_args& g = *p.args;
@@ -61,7 +67,11 @@ float gemm_score(const icl_gemm_variant& v, const GemmProblem& p)
                              ? double(tri_tile_count(g.M, g.N, g.write_tri, v.bm, v.bn))
                              : double((g.M + v.bm - 1) / v.bm) * double((g.N + v.bn - 1) / v.bn);
     const double rate = (v.tflops_hint > 0 ? v.tflops_hint : 1.0) * 1e12;
-    const double t_tile = 2.0 * macs * v.bm * v.bn * double(g.K) * conc / rate;
+    double t_tile = 2.0 * macs * v.bm * v.bn * double(g.K) * conc / rate;
+    if (g.K <= 512) {
+        const double k0 = kTileOverheadK * v.bm * v.bn / (v.threads / 64);
+        t_tile *= (double(g.K) + k0) / double(g.K) * 8192.0 / (8192.0 + k0);
+    }
     const double rounds = std::ceil(tiles / conc);
     const double last = tiles - (rounds - 1.0) * conc;
     double t = (rounds - 1.0) * t_tile + t_tile * std::max(last, kCUs) / conc;

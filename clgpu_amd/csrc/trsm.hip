@@ -24,11 +24,23 @@ struct DiagBlock {
 __device__ __forceinline__ void stage_diag_block(DiagBlock& s, const float* L, long l_si, long l_sk, int nb, int unit,
                                                  int lane)
 {
-    for (int i = 0; i < kNB; ++i) {
-        const int k = lane;
-        float v = 0.0f;
-        if (i < nb && k < nb && k > i) v = L[(long)k * l_si + (long)i * l_sk];
-        s.Lt[i][k] = v;
+    // Sixteen loads in flight at a time (an entry outside the strict lower
+    // triangle reads the block's first diagonal element instead and stores 0).
+#pragma unroll
+    for (int i0 = 0; i0 < kNB; i0 += 16) {
+        float v[16];
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+            const int i = i0 + ii;
+            const bool used = i < nb && lane < nb && lane > i;
+            v[ii] = L[used ? (long)lane * l_si + (long)i * l_sk : 0];
+        }
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+            const int i = i0 + ii;
+            const bool used = i < nb && lane < nb && lane > i;
+            s.Lt[i][lane] = used ? v[ii] : 0.0f;
+        }
     }
     if (lane < nb) s.diag[lane] = unit ? 1.0f : L[(long)lane * l_si + (long)lane * l_sk];
 }
@@ -113,29 +125,30 @@ __global__ void __launch_bounds__(64) trsm_invert_kernel(icl_trsm_blocked_args a
 // total K, and no CU is left with the long tiles while others idle.
 // ---------------------------------------------------------------------------
 constexpr int RT = 64, RK = 32;
-constexpr int kResPer = RT * RK / 256; // elements of each operand slab per thread
 constexpr int kResLd = RT + 16;        // the four 16-lane row groups of a fragment read hit disjoint banks
 
 // One thread's share of a slab of an operand whose element (i, k) (i < RT along
-// the tile, k < RK along K) sits at X[i * s_i + k * s_k]. KFAST: k is the unit
-// stride (s_k == 1), so consecutive lanes take consecutive k. Element s of the
-// thread is (i0 + s * di, k0 + s * dk) with one of di, dk zero.
+// the tile, k < RK along K) sits at X[i * s_i + k * s_k]: eight elements as two
+// runs h of four e along the unit-stride dimension, so that an interior slab of
+// a 16-byte aligned operand is read with two 16-byte loads. KFAST (s_k == 1):
+// (i0 + 32 h, k0 + e); otherwise (s_i == 1): (i0 + e, k0 + 16 h).
 template <bool KFAST>
-struct ResSlab {
-    static constexpr int kDi = KFAST ? 256 / RK : 0, kDk = KFAST ? 0 : 256 / RT;
+struct ResMap {
     int i0, k0;
-    __device__ __forceinline__ explicit ResSlab(int tid)
+    __device__ __forceinline__ explicit ResMap(int tid)
     {
-        i0 = KFAST ? tid / RK : tid % RT;
-        k0 = KFAST ? tid % RK : tid / RT;
+        i0 = KFAST ? tid / 8 : (tid % 16) * 4;
+        k0 = KFAST ? (tid % 8) * 4 : tid / 16;
     }
+    __device__ __forceinline__ int i(int h, int e) const { return KFAST ? i0 + 32 * h : i0 + e; }
+    __device__ __forceinline__ int k(int h, int e) const { return KFAST ? k0 + e : k0 + 16 * h; }
 };
 
 template <bool PT, bool QT>
-__global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
+__global__ void __launch_bounds__(256, 3) residual_f64_kernel(icl_residual_args a)
 {
-    __shared__ float Ps[2][RK][kResLd]; // Ps[b][k][r] = P(m0 + r, kb + k)
-    __shared__ float Qs[2][RK][kResLd]; // Qs[b][k][c] = Q(kb + k, n0 + c)
+    __shared__ __attribute__((aligned(16))) float Ps[2][RK][kResLd]; // Ps[b][k][r] = P(m0 + r, kb + k)
+    __shared__ __attribute__((aligned(16))) float Qs[2][RK][kResLd]; // Qs[b][k][c] = Q(kb + k, n0 + c)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_m = (a.M + RT - 1) / RT, tiles_n = (a.N + RT - 1) / RT;
     const int bid = blockIdx.x;
@@ -158,10 +171,12 @@ __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     // P(i, k) = P[i * ldp + k] when PT, P[i + k * ldp] otherwise; likewise Q(k, c)
     // = Q[k * ldq + c] when QT, Q[c * ldq + k] otherwise (c plays the role of i).
-    const ResSlab<PT> ps(tid);
-    const ResSlab<!QT> qs(tid);
+    const ResMap<PT> pm(tid);
+    const ResMap<!QT> qm(tid);
     const long p_si = PT ? a.ldp : 1, p_sk = PT ? 1 : a.ldp;
     const long q_si = QT ? 1 : a.ldq, q_sk = QT ? a.ldq : 1;
+    const bool p_vec = (reinterpret_cast<uintptr_t>(a.P) & 15) == 0 && (a.ldp & 3) == 0;
+    const bool q_vec = (reinterpret_cast<uintptr_t>(a.Q) & 15) == 0 && (a.ldq & 3) == 0;
 
     for (int t = 0; t < ntiles; ++t) {
         const int m0 = tm[t] * RT, n0 = tn[t] * RT;
@@ -182,23 +197,47 @@ __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
 #pragma unroll
             for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-        const int pi = m0 + ps.i0, qi = n0 + qs.i0;
         // fetch() only loads (an element outside the matrices reads element 0
         // instead), so that all loads of a slab are in flight together; stash()
         // applies the bounds and the triangle rule when it stores to LDS.
-        float pv[kResPer], qv[kResPer];
+        float pv[2][4], qv[2][4];
         auto fetch = [&](int kb) {
-            const float* pp = a.P + (long)pi * p_si + (long)(kb + ps.k0) * p_sk;
-            const float* qp = a.Q + (long)qi * q_si + (long)(kb + qs.k0) * q_sk;
+            const float* pb = a.P + (long)m0 * p_si + (long)kb * p_sk; // P(m0, kb)
+            const float* qb = a.Q + (long)n0 * q_si + (long)kb * q_sk; // Q(kb, n0)
+            const bool k_in = kb + RK <= a.K;
+            if (p_vec && k_in && m0 + RT <= a.M) {
 #pragma unroll
-            for (int s = 0; s < kResPer; ++s) {
-                const int i = pi + s * ps.kDi, k = kb + ps.k0 + s * ps.kDk;
-                pv[s] = *((i < a.M && k < a.K) ? pp + (long)s * (ps.kDi * p_si + ps.kDk * p_sk) : a.P);
+                for (int h = 0; h < 2; ++h) {
+                    const float4 t = *reinterpret_cast<const float4*>(pb + (long)pm.i(h, 0) * p_si +
+                                                                      (long)pm.k(h, 0) * p_sk);
+                    pv[h][0] = t.x, pv[h][1] = t.y, pv[h][2] = t.z, pv[h][3] = t.w;
+                }
+            } else {
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int i = pm.i(h, e), k = pm.k(h, e);
+                        const bool ok = m0 + i < a.M && kb + k < a.K;
+                        pv[h][e] = *(ok ? pb + (long)i * p_si + (long)k * p_sk : a.P);
+                    }
             }
+            if (q_vec && k_in && n0 + RT <= a.N) {
 #pragma unroll
-            for (int s = 0; s < kResPer; ++s) {
-                const int c = qi + s * qs.kDi, k = kb + qs.k0 + s * qs.kDk;
-                qv[s] = *((c < a.N && k < a.K) ? qp + (long)s * (qs.kDi * q_si + qs.kDk * q_sk) : a.Q);
+                for (int h = 0; h < 2; ++h) {
+                    const float4 t = *reinterpret_cast<const float4*>(qb + (long)qm.i(h, 0) * q_si +
+                                                                      (long)qm.k(h, 0) * q_sk);
+                    qv[h][0] = t.x, qv[h][1] = t.y, qv[h][2] = t.z, qv[h][3] = t.w;
+                }
+            } else {
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int c = qm.i(h, e), k = qm.k(h, e);
+                        const bool ok = n0 + c < a.N && kb + k < a.K;
+                        qv[h][e] = *(ok ? qb + (long)c * q_si + (long)k * q_sk : a.Q);
+                    }
             }
         };
         auto stash = [&](int b, int kb) {
@@ -206,20 +245,38 @@ __global__ void __launch_bounds__(256) residual_f64_kernel(icl_residual_args a)
             const bool p_diag = a.tri_on == 1 && kb < m0 + RT && m0 < kb + RK;
             const bool q_diag = a.tri_on == 2 && kb < n0 + RT && n0 < kb + RK;
 #pragma unroll
-            for (int s = 0; s < kResPer; ++s) {
-                const int i = pi + s * ps.kDi, k = kb + ps.k0 + s * ps.kDk;
-                float v = (i < a.M && k < a.K) ? pv[s] : 0.0f;
-                v = (p_diag && (a.tri_lower ? k > i : k < i)) ? 0.0f : v;
-                v = (p_diag && a.tri_unit && k == i) ? 1.0f : v;
-                Ps[b][ps.k0 + s * ps.kDk][ps.i0 + s * ps.kDi] = v;
+            for (int h = 0; h < 2; ++h) {
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int i = m0 + pm.i(h, e), k = kb + pm.k(h, e);
+                    float x = (i < a.M && k < a.K) ? pv[h][e] : 0.0f;
+                    x = (p_diag && (a.tri_lower ? k > i : k < i)) ? 0.0f : x;
+                    v[e] = (p_diag && a.tri_unit && k == i) ? 1.0f : x;
+                }
+                if (PT) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) Ps[b][pm.k(h, e)][pm.i(h, 0)] = v[e];
+                } else {
+                    *reinterpret_cast<float4*>(&Ps[b][pm.k(h, 0)][pm.i(h, 0)]) = make_float4(v[0], v[1], v[2], v[3]);
+                }
             }
 #pragma unroll
-            for (int s = 0; s < kResPer; ++s) { // Q(k, c): row k, column c of the triangle
-                const int c = qi + s * qs.kDi, k = kb + qs.k0 + s * qs.kDk;
-                float v = (c < a.N && k < a.K) ? qv[s] : 0.0f;
-                v = (q_diag && (a.tri_lower ? c > k : c < k)) ? 0.0f : v;
-                v = (q_diag && a.tri_unit && k == c) ? 1.0f : v;
-                Qs[b][qs.k0 + s * qs.kDk][qs.i0 + s * qs.kDi] = v;
+            for (int h = 0; h < 2; ++h) { // Q(k, c): row k, column c of the triangle
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int c = n0 + qm.i(h, e), k = kb + qm.k(h, e);
+                    float x = (c < a.N && k < a.K) ? qv[h][e] : 0.0f;
+                    x = (q_diag && (a.tri_lower ? c > k : c < k)) ? 0.0f : x;
+                    v[e] = (q_diag && a.tri_unit && k == c) ? 1.0f : x;
+                }
+                if (QT) {
+                    *reinterpret_cast<float4*>(&Qs[b][qm.k(h, 0)][qm.i(h, 0)]) = make_float4(v[0], v[1], v[2], v[3]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) Qs[b][qm.k(h, e)][qm.i(h, 0)] = v[e];
+                }
             }
         };
         if (k_lo < k_hi) {
