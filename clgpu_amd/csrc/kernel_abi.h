@@ -250,7 +250,7 @@ hipError_t icl_residual_f64(const icl_residual_args* a, hipStream_t stream);
  * max |X| over block b's share of the m x n matrices D and X (column-major,
  * leading dimensions ldd, ldx), for ICL_ABSMAX_BLOCKS blocks; the host takes the
  * maxima of the partials. NaNs are ignored. */
-#define ICL_ABSMAX_BLOCKS 256
+#define ICL_ABSMAX_BLOCKS 1024
 hipError_t icl_absmax_pair(int m, int n, const float* D, long ldd, const float* X, long ldx, float* partial,
                            hipStream_t stream);
 

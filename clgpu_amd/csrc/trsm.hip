@@ -88,27 +88,72 @@ __global__ void __launch_bounds__(64) trsm_diag_kernel(icl_trsm_diag_args a)
 
 // Inverts every 64 x 64 diagonal block of the canonical lower-triangular system
 // (workgroup b handles rows/columns [64 b, 64 b + nb)): lane j solves
-// L_bb y = e_j in registers, and column j of inv(L_bb) is written to
+// L_bb y = e_j, and column j of inv(L_bb) is written to
 // invD[b * 4096 + j * 64 + i] (column-major, rows and columns beyond nb zero).
-__global__ void __launch_bounds__(64) trsm_invert_kernel(icl_trsm_blocked_args a)
+// The rows of y are split over four waves (wave w owns rows 16 w .. 16 w + 15);
+// per pivot the owning wave divides and publishes y_i through LDS, then every
+// wave updates its later rows -- the recurrence of solve_in_registers, the same
+// operations in the same order, a quarter of them per wave.
+constexpr int kInvRows = kNB / 4;
+__global__ void __launch_bounds__(256) trsm_invert_kernel(icl_trsm_blocked_args a)
 {
     __shared__ DiagBlock s;
-    const int lane = threadIdx.x;
+    __shared__ float piv[2][kNB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = blockIdx.x;
     const int r0 = b * kNB;
     const int nb = min(kNB, a.nsys - r0);
-    stage_diag_block(s, a.L + (long)r0 * (a.l_si + a.l_sk), a.l_si, a.l_sk, nb, a.unit, lane);
+    const float* L = a.L + (long)r0 * (a.l_si + a.l_sk);
+    {
+        // Lt[i][k] = L(k, i) for k > i, rows i of this wave, all loads in flight together
+        float v[kInvRows];
+#pragma unroll
+        for (int ii = 0; ii < kInvRows; ++ii) {
+            const int i = kInvRows * wave + ii;
+            const bool used = i < nb && lane < nb && lane > i;
+            v[ii] = L[used ? (long)lane * a.l_si + (long)i * a.l_sk : 0];
+        }
+#pragma unroll
+        for (int ii = 0; ii < kInvRows; ++ii) {
+            const int i = kInvRows * wave + ii;
+            s.Lt[i][lane] = (i < nb && lane < nb && lane > i) ? v[ii] : 0.0f;
+        }
+        if (wave == 0 && lane < nb) s.diag[lane] = a.unit ? 1.0f : L[(long)lane * (a.l_si + a.l_sk)];
+    }
     __syncthreads();
-    float x[kNB];
+    const int wv = __builtin_amdgcn_readfirstlane(wave); // the same in every lane: scalar branches below
+    const int row0 = kInvRows * wv;
+    float x[kInvRows];
 #pragma unroll
-    for (int i = 0; i < kNB; ++i) x[i] = (i == lane && lane < nb) ? 1.0f : 0.0f;
-    solve_in_registers(x, s, nb, a.unit);
-    float* col = a.invD + (long)b * kNB * kNB + (long)lane * kNB;
+    for (int r = 0; r < kInvRows; ++r) x[r] = (row0 + r == lane && lane < nb) ? 1.0f : 0.0f;
+    for (int ib = 0; ib < kNB; ib += kInvRows) {
 #pragma unroll
-    for (int i = 0; i < kNB; ++i) col[i] = (lane < nb && i < nb) ? x[i] : 0.0f;
+        for (int ii = 0; ii < kInvRows; ++ii) {
+            const int i = ib + ii;
+            if (i >= nb) break;
+            if (wv == ib / kInvRows) {
+                float xi = x[ii];
+                if (!a.unit) xi = xi / s.diag[i];
+                x[ii] = xi;
+                piv[i & 1][lane] = xi;
+            }
+            __syncthreads(); // piv[i & 1] is rewritten two pivots later, after the next barrier
+            const float xi = piv[i & 1][lane];
+            if (wv > ib / kInvRows) {
+#pragma unroll
+                for (int r = 0; r < kInvRows; ++r) x[r] -= s.Lt[i][row0 + r] * xi;
+            } else if (wv == ib / kInvRows) {
+#pragma unroll
+                for (int r = ii + 1; r < kInvRows; ++r) x[r] -= s.Lt[i][row0 + r] * xi;
+            }
+        }
+    }
+    float* col = a.invD + (long)b * kNB * kNB + (long)lane * kNB + row0;
+#pragma unroll
+    for (int r = 0; r < kInvRows; ++r) col[r] = (lane < nb && row0 + r < nb) ? x[r] : 0.0f;
     if (a.sync && b == 0) { // the dataflow solve that follows on this stream starts from zeroed flags
         const int n = ((a.nsys + kNB - 1) / kNB) * ((a.nlines + kNB - 1) / kNB) + 2;
-        for (int i = lane; i < n; i += kNB) a.sync[i] = 0;
+        for (int i = tid; i < n; i += 256) a.sync[i] = 0;
     }
 }
 
@@ -926,7 +971,7 @@ extern "C" hipError_t icl_trsm_invert_blocks(const icl_trsm_blocked_args* a, hip
 {
     if (a->nsys <= 0) return hipSuccess;
     const unsigned blocks = (unsigned)((a->nsys + kNB - 1) / kNB);
-    hipLaunchKernelGGL(trsm_invert_kernel, dim3(blocks), dim3(64), 0, stream, *a);
+    hipLaunchKernelGGL(trsm_invert_kernel, dim3(blocks), dim3(256), 0, stream, *a);
     return hipGetLastError();
 }
 
@@ -1065,12 +1110,29 @@ __global__ void __launch_bounds__(256) absmax_pair_kernel(int m, int n, const fl
                                                           long ldx, float* partial)
 {
     __shared__ float sd[256], sx[256];
-    const long total = (long)m * n;
+    // bpc blocks share a column (more than one when there are fewer columns than
+    // blocks); block b takes part b % bpc of the rows of columns b / bpc,
+    // b / bpc + cstep, ...
+    const int bpc = n >= (int)gridDim.x ? 1 : (int)gridDim.x / n;
+    const int cstep = (int)gridDim.x / bpc, part = blockIdx.x % bpc, c0 = blockIdx.x / bpc;
     float md = 0.0f, mx = 0.0f;
-    for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
-        const long j = e / m, i = e - j * m;
-        md = fmaxf(md, fabsf(D[j * ldd + i]));
-        mx = fmaxf(mx, fabsf(X[j * ldx + i]));
+    if (c0 < cstep) {
+        for (int j = c0; j < n; j += cstep) {
+            const float* dc = D + (long)j * ldd;
+            const float* xc = X + (long)j * ldx;
+            const int step = bpc * 256;
+            int i = part * 256 + (int)threadIdx.x;
+            for (; i + 3 * step < m; i += 4 * step) { // four rows of each matrix in flight
+                const float d0 = dc[i], d1 = dc[i + step], d2 = dc[i + 2 * step], d3 = dc[i + 3 * step];
+                const float x0 = xc[i], x1 = xc[i + step], x2 = xc[i + 2 * step], x3 = xc[i + 3 * step];
+                md = fmaxf(md, fmaxf(fmaxf(fabsf(d0), fabsf(d1)), fmaxf(fabsf(d2), fabsf(d3))));
+                mx = fmaxf(mx, fmaxf(fmaxf(fabsf(x0), fabsf(x1)), fmaxf(fabsf(x2), fabsf(x3))));
+            }
+            for (; i < m; i += step) {
+                md = fmaxf(md, fabsf(dc[i]));
+                mx = fmaxf(mx, fabsf(xc[i]));
+            }
+        }
     }
     sd[threadIdx.x] = md;
     sx[threadIdx.x] = mx;
