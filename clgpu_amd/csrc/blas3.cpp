@@ -68,7 +68,7 @@ float gemm_score(const icl_gemm_variant& v, const GemmProblem& p)
                              : double((g.M + v.bm - 1) / v.bm) * double((g.N + v.bn - 1) / v.bn);
     const double rate = (v.tflops_hint > 0 ? v.tflops_hint : 1.0) * 1e12;
     double t_tile = 2.0 * macs * v.bm * v.bn * double(g.K) * conc / rate;
-    if (g.K <= 512) {
+    if (g.K > 0 && g.K <= 512) { // (a K = 0 call only scales C)
         const double k0 = kTileOverheadK * v.bm * v.bn / (v.threads / 64);
         t_tile *= (double(g.K) + k0) / double(g.K) * 8192.0 / (8192.0 + k0);
     }
