@@ -188,15 +188,14 @@ def ctrsm_rows(h, ib, stream, steps: int) -> dict:
 
 def level3_rows(h, ib, stream, steps: int) -> dict:
     """The GEMM-based level-3 routines on device-resident uniform[-1, 1] data
-    (torch's generator, seed 0), FP32 path: SSYRK, SSYR2K, SSYMM, STRMM at 8192,
-    CHERK, CHEMM, CTRMM at 4096. Useful flops (real flops; a complex
-    multiply-add counts 8): SYRK n^2 k, SYR2K 2 n^2 k, SYMM 2 m^2 n, TRMM m^2 n,
-    HERK 4 n^2 k, HEMM 8 m^2 n, complex TRMM 4 m n^2 (right side). Each result is
-    compared with an FP64 product on the GPU (normwise relative error over the
-    written part)."""
+    (torch's generator, seed 0), on the FP32 path and in the split-BF16 math
+    mode: SSYRK, SSYR2K, SSYMM, STRMM at 8192, CHERK, CHEMM, CTRMM at 4096.
+    Useful flops (real flops; a complex multiply-add counts 8): SYRK n^2 k,
+    SYR2K 2 n^2 k, SYMM 2 m^2 n, TRMM m^2 n, HERK 4 n^2 k, HEMM 8 m^2 n, complex
+    TRMM 4 m n^2 (right side). Each result is compared with an FP64 product on
+    the GPU (normwise relative error over the written part)."""
     import torch
     dev = torch.device("cuda:0")
-    h.set_math_mode(ib.MATH_FP32)
     h.set_implementation(None)
     g = torch.Generator(device=dev)
     g.manual_seed(0)
@@ -217,92 +216,83 @@ def level3_rows(h, ib, stream, steps: int) -> dict:
             d, exp = d * mask, exp * mask
         return float(d.norm() / exp.norm())
 
-    def timed(call, flops):
-        call()
-        torch.cuda.synchronize()
-        ms = _time(stream, call, max(2, steps // 4), 1)
-        tf = flops / (ms * 1e-3) / 1e12
-        return {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
-                "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA}
+    def row(call, flops, err, reset=None):
+        """Both math modes: time the call, then (after reset()) run it once more
+        and take err() of the result."""
+        res = {}
+        for path, mode in (("fp32", ib.MATH_FP32), ("bf16x3", ib.MATH_SPLIT_BF16X3)):
+            h.set_math_mode(mode)
+            call()
+            torch.cuda.synchronize()
+            ms = _time(stream, call, max(2, steps // 4), 1)
+            tf = flops / (ms * 1e-3) / 1e12
+            r = {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
+                 "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA}
+            if reset:
+                reset()
+            call()
+            torch.cuda.synchronize()
+            r["normwise_rel_err"] = err()
+            res[path] = r
+        h.set_math_mode(ib.MATH_FP32)
+        return res
 
     out = {}
     n = 8192
     # SSYRK lower, op N: C = A A^T + 0.5 C
     A, C = rnd(n, n), rnd(n, n)
     C0 = C.clone()
-    call = lambda: ib.check(ib.iclblasSsyrk(h.value, 1, 0, n, n, 1.0, A.data_ptr(), n, 0.5, C.data_ptr(), n))
-    r = timed(call, float(n) * n * n)
-    C.copy_(C0)
-    call()
-    torch.cuda.synchronize()
     low = torch.ones((n, n), device=dev, dtype=torch.float64).tril()
-    r["normwise_rel_err"] = rel(C, (f64(A) @ f64(A).t() + 0.5 * f64(C0)) * low, low)
-    out["ssyrk_8192_k8192_LN"] = r
+    out["ssyrk_8192_k8192_LN"] = row(
+        lambda: ib.check(ib.iclblasSsyrk(h.value, 1, 0, n, n, 1.0, A.data_ptr(), n, 0.5, C.data_ptr(), n)),
+        float(n) * n * n, lambda: rel(C, (f64(A) @ f64(A).t() + 0.5 * f64(C0)) * low, low), lambda: C.copy_(C0))
     # SSYR2K upper, op T, k = 4096
     k = 4096
     A2, B2 = rnd(k, n), rnd(k, n)
     C.copy_(C0)
-    call = lambda: ib.check(ib.iclblasSsyr2k(h.value, 0, 1, n, k, 1.0, A2.data_ptr(), k, B2.data_ptr(), k, 0.0,
-                                             C.data_ptr(), n))
-    r = timed(call, 2.0 * n * n * k)
-    call()
-    torch.cuda.synchronize()
     up = low.t()
     P = f64(A2).t() @ f64(B2)
-    r["normwise_rel_err"] = rel(C, (P + P.t()) * up, up)
-    out["ssyr2k_8192_k4096_UT"] = r
+    out["ssyr2k_8192_k4096_UT"] = row(
+        lambda: ib.check(ib.iclblasSsyr2k(h.value, 0, 1, n, k, 1.0, A2.data_ptr(), k, B2.data_ptr(), k, 0.0,
+                                          C.data_ptr(), n)),
+        2.0 * n * n * k, lambda: rel(C, (P + P.t()) * up, up))
     del A2, B2, P
     # SSYMM left, lower
     Bm = rnd(n, n)
-    call = lambda: ib.check(ib.iclblasSsymm(h.value, 0, 1, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n, 0.0,
-                                            C.data_ptr(), n))
-    r = timed(call, 2.0 * n * n * n)
-    call()
-    torch.cuda.synchronize()
     S = f64(A).tril()
     S = S + S.tril(-1).t()
-    r["normwise_rel_err"] = rel(C, S @ f64(Bm))
-    out["ssymm_8192_LL"] = r
+    out["ssymm_8192_LL"] = row(
+        lambda: ib.check(ib.iclblasSsymm(h.value, 0, 1, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n, 0.0,
+                                         C.data_ptr(), n)),
+        2.0 * n * n * n, lambda: rel(C, S @ f64(Bm)))
     # STRMM left, lower, op N, non-unit (out of place)
-    call = lambda: ib.check(ib.iclblasStrmm(h.value, 0, 1, 0, 0, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n,
-                                            C.data_ptr(), n))
-    r = timed(call, float(n) * n * n)
-    call()
-    torch.cuda.synchronize()
-    r["normwise_rel_err"] = rel(C, f64(A).tril() @ f64(Bm))
-    out["strmm_8192_LLN"] = r
+    out["strmm_8192_LLN"] = row(
+        lambda: ib.check(ib.iclblasStrmm(h.value, 0, 1, 0, 0, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n,
+                                         C.data_ptr(), n)),
+        float(n) * n * n, lambda: rel(C, f64(A).tril() @ f64(Bm)))
     del A, C, C0, Bm, S, low, up
     torch.cuda.empty_cache()
 
     n = 4096
     A, C, Bm = rnd(n, n, True), rnd(n, n, True), rnd(n, n, True)
     # CHERK upper, op N, alpha 1, beta 0
-    call = lambda: ib.check(ib.iclblasCherk(h.value, 0, 0, n, n, 1.0, A.data_ptr(), n, 0.0, C.data_ptr(), n))
-    r = timed(call, 4.0 * n * n * n)
-    call()
-    torch.cuda.synchronize()
     up = torch.ones((n, n), device=dev, dtype=torch.float64).triu().to(torch.complex128)
-    r["normwise_rel_err"] = rel(C, (f64(A) @ f64(A).conj().t()) * up, up)
-    out["cherk_4096_k4096_UN"] = r
+    out["cherk_4096_k4096_UN"] = row(
+        lambda: ib.check(ib.iclblasCherk(h.value, 0, 0, n, n, 1.0, A.data_ptr(), n, 0.0, C.data_ptr(), n)),
+        4.0 * n * n * n, lambda: rel(C, (f64(A) @ f64(A).conj().t()) * up, up))
     # CHEMM left, upper
-    call = lambda: ib.check(ib.iclblasChemm(h.value, 0, 0, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n, 0.0,
-                                            C.data_ptr(), n))
-    r = timed(call, 8.0 * n * n * n)
-    call()
-    torch.cuda.synchronize()
     Hm = f64(A).triu()
     Hm = Hm + Hm.triu(1).conj().t()
     Hm.diagonal().imag.zero_()
-    r["normwise_rel_err"] = rel(C, Hm @ f64(Bm))
-    out["chemm_4096_LU"] = r
+    out["chemm_4096_LU"] = row(
+        lambda: ib.check(ib.iclblasChemm(h.value, 0, 0, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n, 0.0,
+                                         C.data_ptr(), n)),
+        8.0 * n * n * n, lambda: rel(C, Hm @ f64(Bm)))
     # CTRMM right, upper, op C, non-unit
-    call = lambda: ib.check(ib.iclblasCtrmm(h.value, 1, 0, 2, 0, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n,
-                                            C.data_ptr(), n))
-    r = timed(call, 4.0 * n * n * n)
-    call()
-    torch.cuda.synchronize()
-    r["normwise_rel_err"] = rel(C, f64(Bm) @ f64(A).triu().conj().t())
-    out["ctrmm_4096_RUC"] = r
+    out["ctrmm_4096_RUC"] = row(
+        lambda: ib.check(ib.iclblasCtrmm(h.value, 1, 0, 2, 0, n, n, 1.0, A.data_ptr(), n, Bm.data_ptr(), n,
+                                         C.data_ptr(), n)),
+        4.0 * n * n * n, lambda: rel(C, f64(Bm) @ f64(A).triu().conj().t()))
     del A, C, Bm, Hm, up
     torch.cuda.empty_cache()
     return out

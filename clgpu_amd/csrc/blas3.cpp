@@ -190,6 +190,19 @@ void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int 
     run_gemm(h, ICL_KIND_CGEMM, math, opa, opb, g, true);
 }
 
+bool split_depth(iclblasContext* h, int k)
+{
+    return h->math_mode == ICLBLAS_MATH_SPLIT_BF16X3 && k >= kSplitMinK;
+}
+
+void gemm_math_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],
+                   const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc)
+{
+    const int math = h->math_mode == ICLBLAS_MATH_SPLIT_BF16X3 ? ICL_MATH_BF16X3 : ICL_MATH_FP32;
+    if (kind == ICL_KIND_CGEMM) cgemm_dev(h, math, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    else sgemm_dev(h, math, opa, opb, m, n, k, alpha[0], A, lda, B, ldb, beta[0], C, ldc);
+}
+
 void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],
                      const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc,
                      int write_tri, int kband, int cflags)

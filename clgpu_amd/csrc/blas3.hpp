@@ -42,6 +42,16 @@ void sgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int 
 void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int k, const float alpha[2],
                const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc);
 
+// True when the handle runs a GEMM of depth k on the split-BF16 kernels: math
+// mode SPLIT_BF16X3 and k at least the depth from which the split meets the
+// FP32 error bound.
+bool split_depth(iclblasContext* h, int k);
+
+// C = alpha*op(A)*op(B) + beta*C on the kernels of the handle's math mode.
+// kind: ICL_KIND_SGEMM (alpha[1], beta[1] unused) or ICL_KIND_CGEMM.
+void gemm_math_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],
+                   const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc);
+
 // One FP32 MFMA GEMM launch with the level-3 options of icl_gemm_args
 // (write_tri, kband, cflags). kind: ICL_KIND_SGEMM (alpha[1], beta[1] unused) or
 // ICL_KIND_CGEMM. k may be 0: C's selected triangle becomes beta * C and A, B are

@@ -314,6 +314,23 @@ typedef struct icl_expand_args {
 hipError_t icl_expand_real(const icl_expand_args* a, hipStream_t stream);
 hipError_t icl_expand_complex(const icl_expand_args* a, hipStream_t stream);
 
+/* Triangle update of a rank-k result formed densely: for (i, j) in the lower
+ * (i >= j) or upper (i <= j) triangle of the n x n matrix C,
+ *   C(i, j) = alpha * x + beta * C(i, j),  x = T(i, j) (conjugated with
+ *   ICL_CFLAG_CONJ_ACC), C not read when beta == 0, and with ICL_CFLAG_REAL_DIAG
+ *   the diagonal's imaginary part stored as 0;
+ * the other triangle of C is not touched. cplx: interleaved complex data
+ * (ldt, ldc count complex elements). */
+typedef struct icl_tri_update_args {
+    int n, cplx, lower, cflags;
+    const float* T;
+    long ldt;
+    float* C;
+    long ldc;
+    float alpha_re, alpha_im, beta_re, beta_im;
+} icl_tri_update_args;
+hipError_t icl_tri_update(const icl_tri_update_args* a, hipStream_t stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -10,7 +10,9 @@ Chemm, Ssyrk, Csyrk, Cherk, Ssyr2k, Csyr2k, Cher2k, Strmm, Ctrmm):
     diagonal, hermitian diagonal imaginary parts, C when beta == 0), the other
     triangle of C and ld padding compared bit for bit with their input;
   - in-place TRMM (C is B) in both memory modes, and large problems in
-    device memory against float64 numpy products."""
+    device memory against float64 numpy products;
+  - the split-BF16 math mode at depths of at least 256, every variant, within
+    the same bound."""
 import numpy as np
 import pytest
 
@@ -115,6 +117,30 @@ def test_level3_variants_vs_oracle(h, routine, v, shape):
         assert np.array_equal(got[:rc, :cc][other], c["C"][:rc, :cc][other])  # other triangle untouched
     assert np.array_equal(got[rc:], c["C"][rc:])  # ld padding untouched
     assert h.last_implementation().startswith(("sgemm_f32", "cgemm_f32"))
+
+
+@pytest.mark.parametrize("routine,v", ALL, ids=_vid)
+def test_level3_split_bf16_vs_oracle(h, routine, v):
+    """Split-BF16 math mode with every product at least 256 deep: the split
+    kernels (dense SYMM / TRMM products, SYRK-type products into a workspace
+    followed by the triangle update) within the same elementwise bound, the
+    other triangle and the padding untouched."""
+    h.set_math_mode(ib.MATH_SPLIT_BF16X3)
+    m, n, k = 300, 270, 280
+    rng = np.random.default_rng(77)
+    c = random_case(routine, rng, m=m, n=n, k=k, **v)
+    got = _run(h, c)
+    exp, mag, kk = oracle_result(c, c["A"], c["B"], c["C"])
+    (rc, cc) = shapes(c)[2]
+    ok, ratio = _bound_ok(c, got, exp, mag, kk)
+    assert ok, ratio
+    if routine in RANKK or routine in RANK2K:
+        other = np.triu(np.ones((rc, cc), bool), 1) if v["uplo"] == 1 else np.tril(np.ones((rc, cc), bool), -1)
+        assert np.array_equal(got[:rc, :cc][other], c["C"][:rc, :cc][other])
+        if routine in ("Cherk", "Cher2k"):
+            assert np.all(np.diag(got[:rc, :cc]).imag == 0)
+    assert np.array_equal(got[rc:], c["C"][rc:])
+    assert h.last_implementation().startswith(("sgemm_bf16x3", "cgemm_bf16x3"))
 
 
 @pytest.mark.parametrize("routine,v", [(r, v) for r, v in ALL if r not in TRMM][::2], ids=_vid)
