@@ -26,7 +26,15 @@ const std::vector<const icl_gemm_variant*>& gemm_registry()
     return all;
 }
 
-bool gemm_accept(const icl_gemm_variant& v, const GemmProblem& p) { return v.kind == p.kind && v.math == p.math; }
+static bool level3_problem(const icl_gemm_args& g)
+{
+    return g.write_tri != ICL_TRI_NONE || g.kband != ICL_KBAND_NONE || g.cflags != 0;
+}
+
+bool gemm_accept(const icl_gemm_variant& v, const GemmProblem& p)
+{
+    return v.kind == p.kind && v.math == p.math && (v.level3 || !level3_problem(*p.args));
+}
 
 static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
@@ -89,12 +97,12 @@ static bool forced_is_gemm_of_kind(const std::string& name, int kind)
 
 const icl_gemm_variant* gemm_select(iclblasContext* h, const GemmProblem& p)
 {
-    // Problems using the level-3 fields run only on FP32 kernels (the split-BF16
-    // kernels do not implement them), so a forced split variant is not used there.
-    const bool level3 = p.args->write_tri != ICL_TRI_NONE || p.args->kband != ICL_KBAND_NONE || p.args->cflags != 0;
+    // Problems using the level-3 fields run only on kernels that implement them,
+    // so a forced variant without them is not used there.
+    const bool level3 = level3_problem(*p.args);
     if (!h->forced_impl.empty() && forced_is_gemm_of_kind(h->forced_impl, p.kind)) {
         for (const icl_gemm_variant* v : gemm_registry())
-            if (h->forced_impl == v->name && (!level3 || v->math == ICL_MATH_FP32)) return v;
+            if (h->forced_impl == v->name && (!level3 || v->level3)) return v;
     }
     const icl_gemm_variant* best = nullptr;
     float best_score = -1.0f;
@@ -124,8 +132,8 @@ static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, co
     if (math == ICL_MATH_BF16X3 && g.K < kSplitMinK) math = ICL_MATH_FP32;
     GemmProblem p{kind, math, opa, opb, &g};
     const icl_gemm_variant* v = gemm_select(h, p);
-    if ((g.write_tri != ICL_TRI_NONE || g.kband != ICL_KBAND_NONE || g.cflags != 0) && v->math != ICL_MATH_FP32)
-        throw status_error(ICLBLAS_STATUS_INTERNAL_ERROR, "level-3 GEMM options need an FP32 kernel");
+    if (level3_problem(g) && !v->level3)
+        throw status_error(ICLBLAS_STATUS_INTERNAL_ERROR, "level-3 GEMM options on a kernel without them");
     const bool edge = gemm_needs_edge(*v, p);
     void* ws = nullptr;
     if (v->workspace) {
@@ -195,14 +203,6 @@ bool split_depth(iclblasContext* h, int k)
     return h->math_mode == ICLBLAS_MATH_SPLIT_BF16X3 && k >= kSplitMinK;
 }
 
-void gemm_math_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],
-                   const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc)
-{
-    const int math = h->math_mode == ICLBLAS_MATH_SPLIT_BF16X3 ? ICL_MATH_BF16X3 : ICL_MATH_FP32;
-    if (kind == ICL_KIND_CGEMM) cgemm_dev(h, math, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-    else sgemm_dev(h, math, opa, opb, m, n, k, alpha[0], A, lda, B, ldb, beta[0], C, ldc);
-}
-
 void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],
                      const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc,
                      int write_tri, int kband, int cflags)
@@ -225,7 +225,7 @@ void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n
     g.write_tri = write_tri;
     g.kband = kband;
     g.cflags = cflags;
-    run_gemm(h, kind, ICL_MATH_FP32, opa, opb, g, true);
+    run_gemm(h, kind, split_depth(h, k) ? ICL_MATH_BF16X3 : ICL_MATH_FP32, opa, opb, g, true);
 }
 
 // ---------------------------------------------------------------------------

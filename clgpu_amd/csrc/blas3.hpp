@@ -47,13 +47,9 @@ void cgemm_dev(iclblasContext* h, int math, int opa, int opb, int m, int n, int 
 // FP32 error bound.
 bool split_depth(iclblasContext* h, int k);
 
-// C = alpha*op(A)*op(B) + beta*C on the kernels of the handle's math mode.
-// kind: ICL_KIND_SGEMM (alpha[1], beta[1] unused) or ICL_KIND_CGEMM.
-void gemm_math_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],
-                   const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc);
-
-// One FP32 MFMA GEMM launch with the level-3 options of icl_gemm_args
-// (write_tri, kband, cflags). kind: ICL_KIND_SGEMM (alpha[1], beta[1] unused) or
+// One GEMM launch with the level-3 options of icl_gemm_args (write_tri, kband,
+// cflags), on the split-BF16 kernels when split_depth(h, k) and on the FP32 MFMA
+// kernels otherwise. kind: ICL_KIND_SGEMM (alpha[1], beta[1] unused) or
 // ICL_KIND_CGEMM. k may be 0: C's selected triangle becomes beta * C and A, B are
 // not read.
 void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],

@@ -88,7 +88,11 @@ struct CDmaCfg {
     static_assert(WM % 32 == 0 && WN % 32 == 0, "bad wave tile");
 };
 
-template <int BM, int BN, int WVM, int WVN, bool EDGE>
+// L3: the level-3 instantiation -- tiles of the written triangle only
+// (write_tri), the slabs of the tile's k window only (kband), and the epilogue
+// with the triangle, the conjugated accumulator and the real diagonal (cflags);
+// the plain instantiation is compiled without it.
+template <int BM, int BN, int WVM, int WVN, bool EDGE, bool L3 = false>
 __global__ void __launch_bounds__(WVM* WVN * 64) cgemm_bf16x3_dma_kernel(CPackedArgs p)
 {
     using Cfg = CDmaCfg<BM, BN, WVM, WVN>;
@@ -98,8 +102,18 @@ __global__ void __launch_bounds__(WVM* WVN * 64) cgemm_bf16x3_dma_kernel(CPacked
     const int wm0 = (wave % WVM) * Cfg::WM;
     const int wn0 = (wave / WVM) * Cfg::WN;
     int tm, tn;
-    tile_coords<8>(p.Mpad / BM, p.Npad / BN, tm, tn);
+    if constexpr (L3) {
+        if (!tile_coords_level3<BM, BN>(p.g.write_tri, p.g.kband, p.Mpad / BM, p.Npad / BN, tm, tn)) return;
+    } else
+        tile_coords<8>(p.Mpad / BM, p.Npad / BN, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
+    int ks0 = 0, ks1 = p.slabs; // slabs [ks0, ks1) of the tile's k window
+    if constexpr (L3) {
+        int kb = 0, ke = 0;
+        if (!tile_window<BM, BN>(p.g, m0, n0, kb, ke)) return;
+        ks0 = kb / kCPackK;
+        ks1 = ke > kb ? (ke + kCPackK - 1) / kCPackK : ks0;
+    }
 
     const size_t row_elems = 4 * kCPackK;
     const size_t a_slab = (size_t)p.Mpad * row_elems, b_slab = (size_t)p.Npad * row_elems;
@@ -135,15 +149,14 @@ __global__ void __launch_bounds__(WVM* WVN * 64) cgemm_bf16x3_dma_kernel(CPacked
             acci[i][j] = f32x16{};
         }
 
-    const int S = p.slabs;
     const int half = lane >> 5; // which 8-k half of the 16-deep slab this lane supplies
-    if (S > 0) {
-        issue(0, 0);
+    if (ks0 < ks1) {
+        issue(ks0, 0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        for (int ks = 0; ks < S; ++ks) {
-            const int cur = ks & 1;
-            if (ks + 1 < S) issue(ks + 1, cur ^ 1);
+        for (int ks = ks0; ks < ks1; ++ks) {
+            const int cur = (ks - ks0) & 1;
+            if (ks + 1 < ks1) issue(ks + 1, cur ^ 1);
             const unsigned char* As = ldsc + cur * Cfg::STAGE_BYTES;
             const unsigned char* Bs = As + BM * Cfg::ROWB;
             // fragments: [plane][block], plane chunk = 2 * plane + half
@@ -192,8 +205,8 @@ __global__ void __launch_bounds__(WVM* WVN * 64) cgemm_bf16x3_dma_kernel(CPacked
     for (int i = 0; i < Cfg::FM; ++i)
 #pragma unroll
         for (int j = 0; j < Cfg::FN; ++j)
-            store_complex_tile<EDGE>(p.g, accr[i][j], acci[i][j], m0 + wm0 + 32 * i + (lane & 31),
-                                     n0 + wn0 + 32 * j, lane);
+            store_complex_tile<EDGE, L3>(p.g, accr[i][j], acci[i][j], m0 + wm0 + 32 * i + (lane & 31),
+                                         n0 + wn0 + 32 * j, lane);
 }
 
 namespace {
@@ -241,7 +254,13 @@ hipError_t cpacked_launch(const icl_gemm_args* g, int opa, int opb, int edge, vo
     e = cpack(g->B, g->ldb, g->N, g->K, p.Npad, opb == 0, opb == 2, Bp, s);
     if (e != hipSuccess) return e;
     using C = CDmaCfg<BM, BN, WVM, WVN>;
-    if (g->M % BM == 0 && g->N % BN == 0)
+    const bool even = g->M % BM == 0 && g->N % BN == 0;
+    if (uses_level3(g))
+        return even ? launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false, true>,
+                                                              g, s, p)
+                    : launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true, true>,
+                                                              g, s, p);
+    if (even)
         return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, g, s, p);
     return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true>, g, s, p);
 }
@@ -261,7 +280,8 @@ hipError_t cpacked_launch(const icl_gemm_args* g, int opa, int opb, int edge, vo
      icl::cpacked_launch<BM, BN, WVM, WVN>,                                                                            \
      icl::cpacked_workspace<BM, BN>,                                                                                   \
      TF,                                                                                                               \
-     OCC}
+     OCC,                                                                                                              \
+     1}
 
 static const icl_gemm_variant kTable[] = {
     ICL_CGEMM_BF16X3_PACKED("cgemm_bf16x3d_mfma_256x128w8", 256, 128, 4, 2, 403.0f, 1),

@@ -114,16 +114,16 @@ hipError_t cgemm_dispatch(const icl_gemm_args* g, int opa, int opb, int edge, vo
 // TF: measured useful TFLOP/s at m = n = k = 8192 (NN) on one MI355X; OCC: workgroups per CU.
 #define ICL_SGEMM_F32(name, BM, BN, BK, WVM, WVN, TF, OCC)                                                             \
     {name, ICL_KIND_SGEMM, ICL_MATH_FP32, BM, BN, BK, 64 * WVM * WVN,                                                  \
-     icl::sgemm_dispatch<false, BM, BN, BK, WVM, WVN>, nullptr, TF, OCC}
+     icl::sgemm_dispatch<false, BM, BN, BK, WVM, WVN>, nullptr, TF, OCC, 1}
 #define ICL_SGEMM_BF16X3(name, BM, BN, BK, WVM, WVN, TF, OCC)                                                          \
     {name, ICL_KIND_SGEMM, ICL_MATH_BF16X3, BM, BN, BK, 64 * WVM * WVN,                                                \
-     icl::sgemm_dispatch<true, BM, BN, BK, WVM, WVN>, nullptr, TF, OCC}
+     icl::sgemm_dispatch<true, BM, BN, BK, WVM, WVN>, nullptr, TF, OCC, 0}
 #define ICL_CGEMM_F32(name, BM, BN, BK, WVM, WVN, TF, OCC)                                                             \
     {name, ICL_KIND_CGEMM, ICL_MATH_FP32, BM, BN, BK, 64 * WVM * WVN,                                                  \
-     icl::cgemm_dispatch<false, BM, BN, BK, WVM, WVN>, nullptr, TF, OCC}
+     icl::cgemm_dispatch<false, BM, BN, BK, WVM, WVN>, nullptr, TF, OCC, 1}
 #define ICL_CGEMM_BF16X3(name, BM, BN, BK, WVM, WVN, TF, OCC)                                                          \
     {name, ICL_KIND_CGEMM, ICL_MATH_BF16X3, BM, BN, BK, 64 * WVM * WVN,                                                \
-     icl::cgemm_dispatch<true, BM, BN, BK, WVM, WVN>, nullptr, TF, OCC}
+     icl::cgemm_dispatch<true, BM, BN, BK, WVM, WVN>, nullptr, TF, OCC, 0}
 
 #define ICL_REGISTER_VARIANTS(table)                                                                                   \
     namespace {                                                                                                        \

@@ -76,6 +76,7 @@ typedef struct icl_gemm_variant {
     size_t (*workspace)(const icl_gemm_args* args);
     float tflops_hint; /* measured useful TFLOP/s on a full MI355X (m = n = k = 8192, NN) */
     int occupancy;     /* workgroups resident per CU */
+    int level3;        /* implements the level-3 fields write_tri, kband, cflags */
 } icl_gemm_variant;
 
 /* Registered GEMM implementations. Kernel translation units register their
@@ -314,22 +315,6 @@ typedef struct icl_expand_args {
 hipError_t icl_expand_real(const icl_expand_args* a, hipStream_t stream);
 hipError_t icl_expand_complex(const icl_expand_args* a, hipStream_t stream);
 
-/* Triangle update of a rank-k result formed densely: for (i, j) in the lower
- * (i >= j) or upper (i <= j) triangle of the n x n matrix C,
- *   C(i, j) = alpha * x + beta * C(i, j),  x = T(i, j) (conjugated with
- *   ICL_CFLAG_CONJ_ACC), C not read when beta == 0, and with ICL_CFLAG_REAL_DIAG
- *   the diagonal's imaginary part stored as 0;
- * the other triangle of C is not touched. cplx: interleaved complex data
- * (ldt, ldc count complex elements). */
-typedef struct icl_tri_update_args {
-    int n, cplx, lower, cflags;
-    const float* T;
-    long ldt;
-    float* C;
-    long ldc;
-    float alpha_re, alpha_im, beta_re, beta_im;
-} icl_tri_update_args;
-hipError_t icl_tri_update(const icl_tri_update_args* a, hipStream_t stream);
 
 #ifdef __cplusplus
 } /* extern "C" */

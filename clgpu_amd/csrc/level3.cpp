@@ -1,6 +1,8 @@
 // GEMM-based level-3 routines: TRMM, SYMM / HEMM, SYRK / HERK, SYR2K / HER2K.
 //
-// Each runs on the FP32 MFMA GEMM kernels in a single launch per product:
+// Each runs in a single GEMM launch per product, on the FP32 MFMA kernels or,
+// in the split-BF16 math mode when the product is at least the split's minimum
+// depth, on the split-BF16 kernels (both implement the options below):
 //  - SYRK-type updates write one triangle of C. Tiles wholly outside it exit
 //    before loading anything and the epilogue leaves the other triangle
 //    untouched (icl_gemm_args.write_tri); HERK-type updates also store the
@@ -12,12 +14,6 @@
 //    whose tiles each walk only the k range where op(A) can be non-zero
 //    (icl_gemm_args.kband), so the zero half of the flops is skipped except on
 //    the diagonal tiles.
-// In the split-BF16 math mode, when the product's depth reaches the split's
-// minimum, the products run on the split-BF16 kernels instead. Those have no
-// triangle writes or k windows: SYMM / HEMM / TRMM run the plain GEMM over the
-// whole depth (op(A) is 0 outside its triangle), and SYRK-type updates form the
-// dense n x n product in a workspace and write C's triangle from it with
-// icl_tri_update (which also applies the conjugation and the real diagonal).
 //
 // Semantics follow the reference's header contract (include/iclBLAS/iclBLAS.h:
 // Ssymm :1555-1579, Ssyrk :1581-1608, Ssyr2k :1610-1645, Strmm :1721-1753,
@@ -97,16 +93,12 @@ void trmm_dev(iclblasContext* h, bool cplx, int side, int uplo, int trans, int d
         ldo = m;
     }
     const float zero[2] = {0.0f, 0.0f};
-    if (split_depth(h, order)) {
-        if (left) gemm_math_dev(h, kind_of(cplx), 0, 0, m, n, m, alpha, W, order, B, ldb, zero, out, ldo);
-        else gemm_math_dev(h, kind_of(cplx), 0, 0, m, n, n, alpha, B, ldb, W, order, zero, out, ldo);
-    } else if (left) {
+    if (left)
         gemm_level3_dev(h, kind_of(cplx), 0, 0, m, n, m, alpha, W, order, B, ldb, zero, out, ldo, ICL_TRI_NONE,
                         op_lower ? ICL_KBAND_K_BELOW_M_END : ICL_KBAND_K_FROM_M0, 0);
-    } else {
+    else
         gemm_level3_dev(h, kind_of(cplx), 0, 0, m, n, n, alpha, B, ldb, W, order, zero, out, ldo, ICL_TRI_NONE,
                         op_lower ? ICL_KBAND_K_FROM_N0 : ICL_KBAND_K_BELOW_N_END, 0);
-    }
     if (alias)
         hip_check(hipMemcpy2DAsync(C, (size_t)ldc * es, out, (size_t)m * es, (size_t)m * es, (size_t)n,
                                    hipMemcpyDeviceToDevice, h->stream),
@@ -133,16 +125,12 @@ void symm_dev(iclblasContext* h, bool cplx, bool herm, int side, int uplo, int m
     e.lower = uplo == ICLBLAS_FILL_MODE_LOWER ? 1 : 0;
     e.herm = herm ? 1 : 0;
     const float* W = expand(h, cplx, e);
-    if (split_depth(h, order)) {
-        if (left) gemm_math_dev(h, kind_of(cplx), 0, 0, m, n, m, alpha, W, order, B, ldb, beta, C, ldc);
-        else gemm_math_dev(h, kind_of(cplx), 0, 0, m, n, n, alpha, B, ldb, W, order, beta, C, ldc);
-    } else if (left) {
+    if (left)
         gemm_level3_dev(h, kind_of(cplx), 0, 0, m, n, m, alpha, W, order, B, ldb, beta, C, ldc, ICL_TRI_NONE,
                         ICL_KBAND_NONE, 0);
-    } else {
+    else
         gemm_level3_dev(h, kind_of(cplx), 0, 0, m, n, n, alpha, B, ldb, W, order, beta, C, ldc, ICL_TRI_NONE,
                         ICL_KBAND_NONE, 0);
-    }
 }
 
 namespace {
@@ -170,40 +158,6 @@ RankOps rank_ops(bool cplx, bool herm, int trans)
     return r;
 }
 
-// Split-BF16 rank-k (two = false: op(A) op(A)^T) or rank-2k update: the dense
-// n x n product(s) go to a handle workspace T and icl_tri_update writes C's
-// triangle. With a conjugated accumulator alpha must act on the conjugated sum,
-// so the products run with alpha = 1 and tri_update applies alpha; otherwise
-// the products carry alpha (and the second one conj(alpha) for HER2K).
-void rank_update_split(iclblasContext* h, bool cplx, bool herm, bool lower, const RankOps& r, int n, int k,
-                       const float alpha[2], const float* A, long lda, const float* B, long ldb, bool two,
-                       const float beta[2], float* C, long ldc)
-{
-    float* T = static_cast<float*>(
-        h->scratch[iclblasContext::kSlotLevel3Out].get((size_t)n * n * elem_bytes(cplx), h->stream));
-    const bool conj = (r.cflags & ICL_CFLAG_CONJ_ACC) != 0;
-    const float one[2] = {1.0f, 0.0f}, zero[2] = {0.0f, 0.0f};
-    gemm_math_dev(h, kind_of(cplx), r.opa, r.opb, n, n, k, conj ? one : alpha, A, lda, B, ldb, zero, T, n);
-    if (two) {
-        const float alpha2[2] = {alpha[0], herm ? -alpha[1] : alpha[1]};
-        gemm_math_dev(h, kind_of(cplx), r.opa, r.opb, n, n, k, conj ? one : alpha2, B, ldb, A, lda, one, T, n);
-    }
-    icl_tri_update_args u{};
-    u.n = n;
-    u.cplx = cplx ? 1 : 0;
-    u.lower = lower ? 1 : 0;
-    u.cflags = r.cflags;
-    u.T = T;
-    u.ldt = n;
-    u.C = C;
-    u.ldc = ldc;
-    u.alpha_re = conj ? alpha[0] : 1.0f;
-    u.alpha_im = conj ? alpha[1] : 0.0f;
-    u.beta_re = beta[0];
-    u.beta_im = cplx ? beta[1] : 0.0f;
-    hip_check(icl_tri_update(&u, h->stream), "tri_update");
-}
-
 } // namespace
 
 void syrk_dev(iclblasContext* h, bool cplx, bool herm, int uplo, int trans, int n, int k, const float alpha[2],
@@ -213,10 +167,6 @@ void syrk_dev(iclblasContext* h, bool cplx, bool herm, int uplo, int trans, int 
     const int tri = uplo == ICLBLAS_FILL_MODE_LOWER ? ICL_TRI_LOWER : ICL_TRI_UPPER;
     const RankOps r = rank_ops(cplx, herm, trans);
     const int kk = is_zero(cplx, alpha) ? 0 : k; // alpha == 0: A is not read
-    if (kk > 0 && split_depth(h, kk)) {
-        rank_update_split(h, cplx, herm, tri == ICL_TRI_LOWER, r, n, kk, alpha, A, lda, A, lda, false, beta, C, ldc);
-        return;
-    }
     gemm_level3_dev(h, kind_of(cplx), r.opa, r.opb, n, n, kk, alpha, A, lda, A, lda, beta, C, ldc, tri,
                     ICL_KBAND_NONE, r.cflags);
 }
@@ -230,10 +180,6 @@ void syr2k_dev(iclblasContext* h, bool cplx, bool herm, int uplo, int trans, int
     if (k == 0 || is_zero(cplx, alpha)) {
         gemm_level3_dev(h, kind_of(cplx), r.opa, r.opb, n, n, 0, alpha, nullptr, 1, nullptr, 1, beta, C, ldc, tri,
                         ICL_KBAND_NONE, r.cflags);
-        return;
-    }
-    if (split_depth(h, k)) {
-        rank_update_split(h, cplx, herm, tri == ICL_TRI_LOWER, r, n, k, alpha, A, lda, B, ldb, true, beta, C, ldc);
         return;
     }
     // First product with beta, second accumulated on top (beta = 1).

@@ -128,47 +128,7 @@ hipError_t launch_expand(const icl_expand_args* a, hipStream_t stream)
     return hipGetLastError();
 }
 
-// One workgroup per column j of C; its threads walk the rows of the triangle in
-// that column (coalesced along the column). The arithmetic is the FP32 GEMM
-// epilogue's: alpha * x first, then beta * C added with one fma per part.
-template <bool CPLX>
-__global__ void __launch_bounds__(256) tri_update_kernel(icl_tri_update_args a)
-{
-    const int j = blockIdx.x;
-    const int i0 = a.lower ? j : 0, i1 = a.lower ? a.n : j + 1;
-    const bool beta_zero = a.beta_re == 0.0f && (!CPLX || a.beta_im == 0.0f);
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
-        if constexpr (CPLX) {
-            const float2 t = reinterpret_cast<const float2*>(a.T)[(long)j * a.ldt + i];
-            float2* c = reinterpret_cast<float2*>(a.C) + (long)j * a.ldc + i;
-            const float xr = t.x, xi = (a.cflags & ICL_CFLAG_CONJ_ACC) ? -t.y : t.y;
-            float vr = fmaf(a.alpha_re, xr, -a.alpha_im * xi);
-            float vi = fmaf(a.alpha_re, xi, a.alpha_im * xr);
-            if (!beta_zero) {
-                const float2 old = *c;
-                vr += fmaf(a.beta_re, old.x, -a.beta_im * old.y);
-                vi += fmaf(a.beta_re, old.y, a.beta_im * old.x);
-            }
-            if ((a.cflags & ICL_CFLAG_REAL_DIAG) && i == j) vi = 0.0f;
-            *c = make_float2(vr, vi);
-        } else {
-            float* c = a.C + (long)j * a.ldc + i;
-            float v = a.alpha_re * a.T[(long)j * a.ldt + i];
-            if (!beta_zero) v = fmaf(a.beta_re, *c, v);
-            *c = v;
-        }
-    }
-}
-
 } // namespace
-
-extern "C" hipError_t icl_tri_update(const icl_tri_update_args* a, hipStream_t stream)
-{
-    if (a->n <= 0) return hipSuccess;
-    if (a->cplx) hipLaunchKernelGGL(tri_update_kernel<true>, dim3((unsigned)a->n), dim3(256), 0, stream, *a);
-    else hipLaunchKernelGGL(tri_update_kernel<false>, dim3((unsigned)a->n), dim3(256), 0, stream, *a);
-    return hipGetLastError();
-}
 
 extern "C" hipError_t icl_expand_real(const icl_expand_args* a, hipStream_t stream)
 {

@@ -118,7 +118,10 @@ struct DmaCfg {
     static_assert(WM % 32 == 0 && WN % 32 == 0, "bad wave tile");
 };
 
-template <int BM, int BN, int WVM, int WVN, bool EDGE>
+// L3: the level-3 instantiation -- tiles of the written triangle only
+// (write_tri), the slabs of the tile's k window only (kband), and the epilogue
+// that stores only the triangle; the plain instantiation is compiled without it.
+template <int BM, int BN, int WVM, int WVN, bool EDGE, bool L3 = false>
 __global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_dma_kernel(PackedArgs p)
 {
     using Cfg = DmaCfg<BM, BN, WVM, WVN>;
@@ -128,8 +131,18 @@ __global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_dma_kernel(PackedA
     const int wm0 = (wave % WVM) * Cfg::WM;
     const int wn0 = (wave / WVM) * Cfg::WN;
     int tm, tn;
-    tile_coords<8>(p.Mpad / BM, p.Npad / BN, tm, tn);
+    if constexpr (L3) {
+        if (!tile_coords_level3<BM, BN>(p.g.write_tri, p.g.kband, p.Mpad / BM, p.Npad / BN, tm, tn)) return;
+    } else
+        tile_coords<8>(p.Mpad / BM, p.Npad / BN, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
+    int ks0 = 0, ks1 = p.slabs; // slabs [ks0, ks1) of the tile's k window
+    if constexpr (L3) {
+        int kb = 0, ke = 0;
+        if (!tile_window<BM, BN>(p.g, m0, n0, kb, ke)) return;
+        ks0 = kb / kPackK;
+        ks1 = ke > kb ? (ke + kPackK - 1) / kPackK : ks0;
+    }
 
     const size_t a_slab = (size_t)p.Mpad * (2 * kPackK);
     const size_t b_slab = (size_t)p.Npad * (2 * kPackK);
@@ -162,14 +175,13 @@ __global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_dma_kernel(PackedA
 #pragma unroll
         for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = f32x16{};
 
-    const int S = p.slabs;
-    if (S > 0) {
-        issue(0, 0);
+    if (ks0 < ks1) {
+        issue(ks0, 0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        for (int ks = 0; ks < S; ++ks) {
-            const int cur = ks & 1;
-            if (ks + 1 < S) issue(ks + 1, cur ^ 1);
+        for (int ks = ks0; ks < ks1; ++ks) {
+            const int cur = (ks - ks0) & 1;
+            if (ks + 1 < ks1) issue(ks + 1, cur ^ 1);
             const unsigned char* As = ldsd + cur * Cfg::STAGE_BYTES;
             const unsigned char* Bs = As + BM * Cfg::ROWB;
 #pragma unroll
@@ -211,7 +223,7 @@ __global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_dma_kernel(PackedA
     for (int i = 0; i < Cfg::FM; ++i)
 #pragma unroll
         for (int j = 0; j < Cfg::FN; ++j)
-            store_real_tile<EDGE>(p.g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
+            store_real_tile<EDGE, L3>(p.g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
 }
 
 namespace {
@@ -256,6 +268,11 @@ hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, voi
     (void)edge;
     // launch_gemm sizes the grid from (M, N); the packed grid is the same.
     using C = DmaCfg<BM, BN, WVM, WVN>;
+    if (uses_level3(g))
+        return even ? launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false, true>,
+                                                              g, s, p)
+                    : launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true, true>,
+                                                              g, s, p);
     if (even)
         return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, g, s, p);
     return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true>, g, s, p);
@@ -276,7 +293,8 @@ hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, voi
      icl::packed_launch<BM, BN, WVM, WVN>,                                                                             \
      icl::packed_workspace<BM, BN>,                                                                                    \
      TF,                                                                                                               \
-     OCC}
+     OCC,                                                                                                              \
+     1}
 
 static const icl_gemm_variant kTable[] = {
     ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8", 256, 256, 4, 2, 406.2f, 1),

@@ -122,9 +122,9 @@ def test_level3_variants_vs_oracle(h, routine, v, shape):
 @pytest.mark.parametrize("routine,v", ALL, ids=_vid)
 def test_level3_split_bf16_vs_oracle(h, routine, v):
     """Split-BF16 math mode with every product at least 256 deep: the split
-    kernels (dense SYMM / TRMM products, SYRK-type products into a workspace
-    followed by the triangle update) within the same elementwise bound, the
-    other triangle and the padding untouched."""
+    kernels' level-3 instantiation (triangle tiles and stores, k windows,
+    conjugated accumulator, real diagonal) within the same elementwise bound,
+    the other triangle and the padding untouched."""
     h.set_math_mode(ib.MATH_SPLIT_BF16X3)
     m, n, k = 300, 270, 280
     rng = np.random.default_rng(77)
