@@ -10,10 +10,12 @@
  *    device pointers (hipMalloc / torch CUDA tensors), launches on the
  *    handle's HIP stream and returns without waiting. Scalars (alpha, beta)
  *    are host pointers in both modes.
- *  - Math mode: ICLBLAS_MATH_FP32 runs SGEMM/CGEMM on FP32 MFMA.
- *    ICLBLAS_MATH_SPLIT_BF16X3 splits every FP32 operand x into
+ *  - Math mode: ICLBLAS_MATH_FP32 runs SGEMM/CGEMM and the level-3 routines on
+ *    FP32 MFMA. ICLBLAS_MATH_SPLIT_BF16X3 splits every FP32 operand x into
  *    hi = bf16(x), lo = bf16(x - hi) and forms hi*hi + hi*lo + lo*hi on BF16
- *    MFMA with FP32 accumulation.
+ *    MFMA with FP32 accumulation, for products at least 256 deep (shallower
+ *    ones stay on FP32 MFMA); the error stays within the FP32 bound
+ *    4 k 2^-24 (|A||B|).
  *  - Refined STRSM: FP32 blocked solve followed by sweeps of iterative
  *    refinement whose residual R = alpha*B - op(A)*X is formed in FP64.
  *  - Adjoints of SGEMM and STRSM.
