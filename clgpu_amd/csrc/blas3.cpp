@@ -763,6 +763,7 @@ void trsm_check_error(iclblasContext* h)
 void strsm_refined_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
                        const float* A, long lda, float* B, long ldb, int sweeps, float* B0, float* R)
 {
+    h->last_sweeps = 0;
     if (m <= 0 || n <= 0) return;
     if (alpha == 0.0f || sweeps <= 0) {
         strsm_dev(h, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb);

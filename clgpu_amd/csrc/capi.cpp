@@ -187,6 +187,7 @@ static iclblasStatus_t strsm_common(iclblasHandle_t handle, iclblasSideMode_t si
     return guarded([&] {
         HandleLock lock(handle);
         iclblasContext* h = handle;
+        h->last_sweeps = 0; // a call without sweeps (or with alpha == 0) ran none
         const float a = *alpha;
         const int order = side == ICLBLAS_SIDE_LEFT ? m : n;
         const float* dA = A;

@@ -74,7 +74,8 @@ ICLBLAS_API iclblasStatus_t iclblasStrsmRefined(iclblasHandle_t handle, iclblasS
                                                 iclblasFillMode_t uplo, iclblasOperation_t trans,
                                                 iclblasDiagType_t diag, int m, int n, const float* alpha,
                                                 float* A, int lda, float* B, int ldb, int max_sweeps);
-/* Number of refinement sweeps the handle's last iclblasStrsmRefined ran. */
+/* Number of refinement sweeps the handle's last STRSM call ran (0 after
+ * iclblasStrsm, max_sweeps == 0 or alpha == 0). */
 ICLBLAS_API iclblasStatus_t iclblasGetLastRefinementSweeps(iclblasHandle_t handle, int* sweeps);
 
 /* Adjoint of C = alpha*op(A)*op(B) + beta*C for an upstream gradient dC (m x n):

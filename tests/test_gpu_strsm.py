@@ -109,6 +109,20 @@ def test_strsm_refined_variants(h, v, shape):
     assert rel <= 2.0 ** -23, rel
 
 
+def test_strsm_refined_sweep_count_resets(h):
+    """A refined call with no sweeps (or alpha == 0) reports 0 sweeps, whatever
+    the handle's previous call ran."""
+    m, n = 190, 45
+    A, B = _problem(0, 1, 0, m, n, seed=46)
+    X = B.copy(order="F")
+    ib.check(ib.iclblasStrsmRefined(h.value, 0, 1, 0, 0, m, n, 1.0, A, A.shape[0], X, X.shape[0], 2))
+    assert h.last_refinement_sweeps() >= 1
+    for sweeps, alpha in ((0, 1.0), (2, 0.0)):
+        X = B.copy(order="F")
+        ib.check(ib.iclblasStrsmRefined(h.value, 0, 1, 0, 0, m, n, alpha, A, A.shape[0], X, X.shape[0], sweeps))
+        assert h.last_refinement_sweeps() == 0
+
+
 def _to_dev(a):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
