@@ -1,6 +1,8 @@
-// Complex triangular solve of one diagonal block (at most 64 x 64) for CTRSM.
+// CTRSM kernels: the solve of one diagonal block (at most 64 x 64), the
+// inversion of every 64 x 64 diagonal block, the 256 x 256 panel inverses built
+// from those, and the in-place product of a leaf with its block inverse.
 //
-// The arithmetic is the reference kernel's (libs/blas/funcs/Ctrsm/Ctrsm_naive.cl:16-183
+// The substitution's arithmetic is the reference kernel's (libs/blas/funcs/Ctrsm/Ctrsm_naive.cl:16-183
 // with the helpers of core/src/ocl/complex.h.cl:32-39): per pivot i,
 // x_i = cdiv(b_i, a_ii) for a non-unit diagonal, then b_k -= cmul(a_ki, x_i) for
 // every later k, and the solved value is stored as cmul(x_i, alpha). OP_C

@@ -1,5 +1,6 @@
-// STRSM building blocks for gfx950: the diagonal-block solver, the FP64
-// residual used by iterative refinement, and 2-D elementwise updates.
+// STRSM building blocks for gfx950: the diagonal-block solver and inverter,
+// the blocked / dataflow / panel solves, the FP64 residual and stopping test
+// of iterative refinement, and 2-D elementwise updates.
 //
 // The substitution follows the reference kernel's per-column recurrence
 // (libs/blas/funcs/Strsm/Strsm_naive.cl:28-37): for pivot i, x_i = b_i / a_ii
