@@ -108,6 +108,7 @@ ICLBLAS_API iclblasStatus_t iclblasDestroy(iclblasHandle_t handle)
     if (!iclblasContext::valid(handle)) return ICLBLAS_STATUS_INVALID_VALUE;
     (void)hipSetDevice(handle->device);
     (void)hipStreamSynchronize(handle->stream);
+    handle->copy.release();
     for (auto& s : handle->scratch) s.release();
     if (handle->trsm_error) (void)hipFree(handle->trsm_error);
     if (handle->own_stream) (void)hipStreamDestroy(handle->stream);
@@ -131,6 +132,13 @@ ICLBLAS_API iclblasStatus_t iclblasSgemm(iclblasHandle_t handle, iclblasOperatio
         const int rb = transb == ICLBLAS_OP_N ? k : n, cb = transb == ICLBLAS_OP_N ? n : k;
         if (h->memory_mode == ICLBLAS_MEMORY_DEVICE) {
             icl::sgemm_dev(h, math_of(h), transa, transb, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+        } else if (const int width = host_gemm_panel_width(h, m, n, k)) {
+            const bool use_ab = k > 0 && a != 0.0f;
+            host_gemm_panels<float>(h, transb, m, n, k, use_ab, b != 0.0f, A, ra, ca, lda, B, rb, cb, ldb, C, ldc, width,
+                                    [&](const float* dA, const float* dB, float* dC, int, int w) {
+                                        icl::sgemm_dev(h, math_of(h), transa, transb, m, w, k, a, dA, lda, dB, ldb, b,
+                                                       dC, ldc);
+                                    });
         } else {
             const bool use_ab = k > 0 && a != 0.0f;
             const float* dA = use_ab ? stage_in<float>(h, kSlotA, A, ra, ca, lda, true) : nullptr;
@@ -162,6 +170,17 @@ ICLBLAS_API iclblasStatus_t iclblasCgemm(iclblasHandle_t handle, iclblasOperatio
         if (h->memory_mode == ICLBLAS_MEMORY_DEVICE) {
             icl::cgemm_dev(h, math_of(h), transa, transb, m, n, k, a, reinterpret_cast<const float*>(A), lda,
                            reinterpret_cast<const float*>(B), ldb, b, reinterpret_cast<float*>(C), ldc);
+        } else if (const int width = host_gemm_panel_width(h, m, n, k)) {
+            const bool use_ab = k > 0 && (a[0] != 0.0f || a[1] != 0.0f);
+            const bool use_c = b[0] != 0.0f || b[1] != 0.0f;
+            host_gemm_panels<cf>(h, transb, m, n, k, use_ab, use_c, reinterpret_cast<const cf*>(A), ra, ca, lda,
+                                 reinterpret_cast<const cf*>(B), rb, cb, ldb, reinterpret_cast<cf*>(C), ldc, width,
+                                 [&](const cf* dA, const cf* dB, cf* dC, int, int w) {
+                                     icl::cgemm_dev(h, math_of(h), transa, transb, m, w, k, a,
+                                                    reinterpret_cast<const float*>(dA), lda,
+                                                    reinterpret_cast<const float*>(dB), ldb, b,
+                                                    reinterpret_cast<float*>(dC), ldc);
+                                 });
         } else {
             const bool use_ab = k > 0 && (a[0] != 0.0f || a[1] != 0.0f);
             const bool use_c = b[0] != 0.0f || b[1] != 0.0f;

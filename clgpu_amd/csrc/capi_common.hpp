@@ -72,6 +72,61 @@ void stage_out(iclblasContext* h, T* host, const T* dev, int rows, int cols, int
     h->stager.d2h(host, dev, rows, cols, ld, sizeof(T), h->stream);
 }
 
+// Host-pointer GEMM with the transfers overlapped with the compute, for large
+// problems. C is done in column panels: A (and op(B) when its columns are not
+// contiguous in B, i.e. op T / C) goes up first on the handle's copy lane, then
+// panel j's columns of B (op N) and of C (beta != 0) go up while panel j - 1 is
+// computed on the handle's stream, and panel j - 1 comes back while panel j is
+// computed. One event per direction and panel orders the two streams.
+// compute(dA, dB, dC, c0, w) launches the GEMM of C's columns [c0, c0 + w)
+// (dB and dC already point at the panel's first column of op(B) and C).
+template <typename T, typename Compute>
+void host_gemm_panels(iclblasContext* h, int opb, int m, int n, int k, bool use_ab, bool use_c, const T* A, int ra,
+                      int ca, int lda, const T* B, int rb, int cb, int ldb, T* C, int ldc, int width, Compute compute)
+{
+    h->copy.ensure();
+    const hipStream_t cs = h->copy.stream;
+    T* dA = use_ab ? static_cast<T*>(h->scratch[kSlotA].get(extent(ra, ca, lda) * sizeof(T), h->stream)) : nullptr;
+    T* dB = use_ab ? static_cast<T*>(h->scratch[kSlotB].get(extent(rb, cb, ldb) * sizeof(T), h->stream)) : nullptr;
+    T* dC = static_cast<T*>(h->scratch[kSlotC].get(extent(m, n, ldc) * sizeof(T), h->stream));
+    const bool b_cols = opb == ICLBLAS_OP_N; // op(B)'s columns are B's columns
+    if (use_ab) h->stager.h2d(dA, A, extent(ra, ca, lda) * sizeof(T), cs);
+    if (use_ab && !b_cols) h->stager.h2d(dB, B, extent(rb, cb, ldb) * sizeof(T), cs);
+    constexpr int E = CopyLane::kEvents;
+    int prev_c0 = 0, prev_w = 0, j = 0;
+    for (int c0 = 0; c0 < n; c0 += width, ++j) {
+        const int w = n - c0 < width ? n - c0 : width;
+        if (use_ab && b_cols) h->stager.h2d(dB + (size_t)c0 * ldb, B + (size_t)c0 * ldb, extent(k, w, ldb) * sizeof(T), cs);
+        if (use_c) h->stager.h2d(dC + (size_t)c0 * ldc, C + (size_t)c0 * ldc, extent(m, w, ldc) * sizeof(T), cs);
+        hip_check(hipEventRecord(h->copy.in[j % E], cs), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(h->stream, h->copy.in[j % E], 0), "hipStreamWaitEvent");
+        const T* dBp = use_ab ? (b_cols ? dB + (size_t)c0 * ldb : dB + c0) : nullptr;
+        compute(dA, dBp, dC + (size_t)c0 * ldc, c0, w);
+        hip_check(hipEventRecord(h->copy.out[j % E], h->stream), "hipEventRecord");
+        if (j > 0) {
+            hip_check(hipStreamWaitEvent(cs, h->copy.out[(j - 1) % E], 0), "hipStreamWaitEvent");
+            h->stager.d2h(C + (size_t)prev_c0 * ldc, dC + (size_t)prev_c0 * ldc, m, prev_w, ldc, sizeof(T), cs);
+        }
+        prev_c0 = c0;
+        prev_w = w;
+    }
+    hip_check(hipStreamWaitEvent(cs, h->copy.out[(j - 1) % E], 0), "hipStreamWaitEvent");
+    h->stager.d2h(C + (size_t)prev_c0 * ldc, dC + (size_t)prev_c0 * ldc, m, prev_w, ldc, sizeof(T), cs);
+}
+
+// Column-panel width for host_gemm_panels, or 0 when the overlap does not pay
+// and one transfer per matrix is cheaper: small problems, and the split-BF16
+// math mode, whose compute is short next to the transfers (measured at 8192^3:
+// 20.8 ms pipelined against 19.7 ms, where the FP32 path gains 25.9 -> 21.3 ms).
+inline int host_gemm_panel_width(const iclblasContext* h, int m, int n, int k)
+{
+    if (h->math_mode == ICLBLAS_MATH_SPLIT_BF16X3) return 0;
+    if ((double)m * n * k < 4294967296.0 || n < 2048) return 0;
+    int w = (n + 7) / 8;         // about eight panels
+    w = (w + 255) / 256 * 256;   // whole 256-column tiles
+    return w;
+}
+
 inline void finish(iclblasContext* h)
 {
     if (h->memory_mode == ICLBLAS_MEMORY_HOST) icl::hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");

@@ -97,6 +97,43 @@ private:
     size_t cap_ = 0;
 };
 
+// A stream for host <-> device copies that overlap the kernels on the handle's
+// stream (non-blocking, default priority), with events that order the two per
+// pipeline step (reused round robin). Created on first use.
+struct CopyLane {
+    static constexpr int kEvents = 16;
+    hipStream_t stream = nullptr;
+    hipEvent_t in[kEvents] = {}, out[kEvents] = {};
+    CopyLane() = default;
+    CopyLane(const CopyLane&) = delete;
+    CopyLane& operator=(const CopyLane&) = delete;
+    ~CopyLane() { release(); }
+
+    void ensure()
+    {
+        if (stream) return;
+        for (int i = 0; i < kEvents; ++i) {
+            hip_check(hipEventCreateWithFlags(&in[i], hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventCreateWithFlags(&out[i], hipEventDisableTiming), "hipEventCreate");
+        }
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    }
+
+    void release()
+    {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+            stream = nullptr;
+        }
+        for (int i = 0; i < kEvents; ++i) {
+            if (in[i]) (void)hipEventDestroy(in[i]);
+            if (out[i]) (void)hipEventDestroy(out[i]);
+            in[i] = out[i] = nullptr;
+        }
+    }
+};
+
 } // namespace icl
 
 struct iclblasContext {
@@ -121,6 +158,7 @@ struct iclblasContext {
     std::mutex mu; // a handle serializes its own calls
     icl::DeviceScratch scratch[kScratchSlots];
     icl::PinnedStager stager;
+    icl::CopyLane copy;
     int* trsm_error = nullptr; // device word set by an aborted dataflow STRSM (allocated on first use)
 
     static bool valid(const iclblasContext* h) { return h != nullptr && h->tag == kTag; }

@@ -145,3 +145,33 @@ def test_cgemm_config3_4096(dev_handle, mode, ta):
     extra = 4 * U32 * (abs(beta) * np.abs(C[rows, cols]).astype(np.float64) + abs(alpha) * ab)
     assert bound_ratio(np.abs(got - c64), n, abs(alpha) * ab, extra) <= 4.0
     h.set_math_mode(ib.MATH_FP32)
+
+
+@pytest.mark.parametrize("tb", [0, 2], ids=["BN", "BC"])
+@pytest.mark.parametrize("beta", [0.0, 0.5 - 0.25j], ids=["beta0", "beta"])
+def test_cgemm_host_panel_pipeline(h, tb, beta):
+    """Host pointers at a size that takes the panel pipeline (B and C in column
+    panels, transfers overlapped with the compute, a partial last panel), padded
+    leading dimensions, B as is and conjugate-transposed, C not read when
+    beta == 0; against a complex128 product within the elementwise bound."""
+    m, n, k = 1024, 2304, 2048
+    rb, cb = (k, n) if tb == 0 else (n, k)
+    A = datagen.complex_matrix(m, k, m + 1, "A", seed=32)
+    B = datagen.complex_matrix(rb, cb, rb + 3, "B", seed=32)
+    C = datagen.complex_matrix(m, n, m + 2, "C", seed=32)
+    C[m:] = -7.0 + 1.0j
+    if beta == 0.0:
+        C[:m] = np.nan
+    C0 = C.copy(order="F")
+    alpha = 0.5 + 0.75j
+    run(h, 0, tb, m, n, k, alpha, A, B, beta, C)
+    A64 = A[:m].astype(np.complex128)
+    opB = B[:rb].astype(np.complex128) if tb == 0 else B[:rb].astype(np.complex128).conj().T
+    ref = alpha * (A64 @ opB)
+    mag = abs(alpha) * (np.abs(A64) @ np.abs(opB))
+    if beta != 0.0:
+        ref += beta * C0[:m].astype(np.complex128)
+        mag += abs(beta) * np.abs(C0[:m].astype(np.complex128))
+    err = np.abs(C[:m].astype(np.complex128) - ref)
+    assert np.all(err <= 8 * (k + 2) * U32 * mag), float((err / np.maximum(mag, 1e-30)).max())
+    assert np.all(C[m:] == np.complex64(-7.0 + 1.0j))

@@ -246,3 +246,36 @@ def test_sgemm_config2_nonsquare_padded(dev_handle, mode):
     torch.cuda.synchronize()
     assert bool((dC2.t()[:m] == Ct[:m]).all())  # 2 * (A (B/2)) is bitwise A B (power-of-two scaling)
     h.set_math_mode(ib.MATH_FP32)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("tb", [0, 1], ids=["BN", "BT"])
+@pytest.mark.parametrize("beta", [0.0, -0.5], ids=["beta0", "beta"])
+def test_sgemm_host_panel_pipeline(h, mode, tb, beta):
+    """Host pointers at a size where the FP32 math mode takes the panel
+    pipeline (B and C in column panels, transfers overlapped with the compute, a
+    partial last panel) and the split mode one transfer per matrix; padded
+    leading dimensions, both layouts of B, C not read when beta == 0; against a
+    float64 product within the elementwise bound."""
+    m, n, k = 2048, 2304, 1024
+    h.set_math_mode(mode)
+    rb, cb = (k, n) if tb == 0 else (n, k)
+    A = datagen.matrix(m, k, m + 3, "A", seed=31)
+    B = datagen.matrix(rb, cb, rb + 5, "B", seed=31)
+    C = datagen.matrix(m, n, m + 7, "C", seed=31)
+    C[m:] = -7.0
+    if beta == 0.0:
+        C[:m] = np.nan
+    C0 = C.copy(order="F")
+    alpha = 0.75
+    run(h, 0, tb, m, n, k, alpha, A, B, beta, C)
+    A64 = A[:m].astype(np.float64)
+    opB = B[:rb].astype(np.float64) if tb == 0 else B[:rb].astype(np.float64).T
+    ref = alpha * (A64 @ opB)
+    mag = abs(alpha) * (np.abs(A64) @ np.abs(opB))
+    if beta != 0.0:
+        ref += beta * C0[:m].astype(np.float64)
+        mag += abs(beta) * np.abs(C0[:m].astype(np.float64))
+    err = np.abs(C[:m].astype(np.float64) - ref)
+    assert np.all(err <= 4 * (k + 2) * U32 * mag), float((err / np.maximum(mag, 1e-30)).max())
+    assert np.all(C[m:] == -7.0)
