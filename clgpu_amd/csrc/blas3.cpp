@@ -244,6 +244,33 @@ constexpr int kLeaf = 64;
 constexpr bool kTrsmPanelDefault = true;
 constexpr int kPanel = 256;
 
+// X_p = W_p B_p (left: the panel's rows of B, all columns) or B_p W_p (right:
+// the panel's columns of B, all rows), as the arguments of icl_panel_apply. W_p
+// is plen x plen with leading dimension kPanel; X is m x n with leading
+// dimension m; p_lo is the panel's first unknown in original order; pointers
+// and strides count complex elements when cplx.
+icl_panel_apply_args panel_product(bool cplx, bool left, int plen, int m, int n, const float* Wp, const float* B,
+                                   long ldb, float* X, int p_lo)
+{
+    icl_panel_apply_args a{};
+    const long es = cplx ? 2 : 1;
+    a.cplx = cplx ? 1 : 0;
+    a.p = plen;
+    a.W = Wp;
+    if (left) { // X(r, l) = sum_k W(r, k) B(k, l), lines = columns
+        a.nlines = n;
+        a.w_rs = 1, a.w_ks = kPanel;
+        a.B = B + es * p_lo, a.b_ks = 1, a.b_ls = ldb;
+        a.X = X + es * p_lo, a.x_rs = 1, a.x_ls = m;
+    } else { // X(l, c) = sum_k B(l, k) W(k, c), lines = rows
+        a.nlines = m;
+        a.w_rs = kPanel, a.w_ks = 1;
+        a.B = B + es * (long)p_lo * ldb, a.b_ks = ldb, a.b_ls = 1;
+        a.X = X + es * (long)p_lo * m, a.x_rs = m, a.x_ls = 1;
+    }
+    return a;
+}
+
 struct Trsm {
     iclblasContext* h;
     bool left, trans, unit, fwd;
@@ -545,12 +572,8 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
         const int p_lo = t.fwd ? ps : t.order - pe; // original index of the panel's first unknown
         const int r_lo = t.fwd ? pe : 0;            // original index of the first later unknown
         const float* Wp = W + (size_t)(ps / kPanel) * kPanel * kPanel * 2;
-        if (t.left)
-            cgemm_dev(h, ICL_MATH_FP32, 0, 0, plen, n, plen, one, Wp, kPanel, B + 2 * (long)p_lo, ldb, zero,
-                      Xb + 2 * (long)p_lo, m);
-        else
-            cgemm_dev(h, ICL_MATH_FP32, 0, 0, m, plen, plen, one, B + 2 * (long)p_lo * ldb, ldb, Wp, kPanel, zero,
-                      Xb + 2 * (long)p_lo * m, m);
+        const icl_panel_apply_args xa = panel_product(true, t.left, plen, m, n, Wp, B, ldb, Xb, p_lo);
+        hip_check(icl_panel_apply(&xa, h->stream), "panel_apply");
         if (pe == t.order) break;
         int op = 0;
         if (t.left) {
@@ -689,12 +712,8 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
                 const int r_lo = t.fwd ? pe : 0;          // original index of the first later unknown
                 if (plen == kPanel && ps / kPanel < nfull) {
                     const float* Wp = W + (size_t)(ps / kPanel) * kPanel * kPanel;
-                    if (t.left)
-                        sgemm_dev(h, ICL_MATH_FP32, 0, 0, kPanel, n, kPanel, 1.0f, Wp, kPanel, B + p_lo, ldb, 0.0f,
-                                  Xb + p_lo, m, ICL_MASK_NONE, false);
-                    else
-                        sgemm_dev(h, ICL_MATH_FP32, 0, 0, m, kPanel, kPanel, 1.0f, B + (long)p_lo * ldb, ldb, Wp, kPanel,
-                                  0.0f, Xb + (long)p_lo * m, m, ICL_MASK_NONE, false);
+                    const icl_panel_apply_args pa = panel_product(false, t.left, kPanel, m, n, Wp, B, ldb, Xb, p_lo);
+                    hip_check(icl_panel_apply(&pa, h->stream), "panel_apply");
                 } else {
                     hip_check(icl_trsm_panel(&a, ps, pe, h->stream), "trsm_panel");
                 }

@@ -134,6 +134,22 @@ typedef struct icl_ctrsm_blocked_args {
 } icl_ctrsm_blocked_args;
 hipError_t icl_ctrsm_invert_blocks(const icl_ctrsm_blocked_args* a, hipStream_t stream);
 
+/* X = W * B for one diagonal panel of a blocked TRSM: X(r, l) = sum_k W(r, k)
+ * B(k, l) for r, k < p (p <= 256) and l < nlines, with W(r, k) = W[r * w_rs +
+ * k * w_ks], B(k, l) = B[k * b_ks + l * b_ls], X(r, l) = X[r * x_rs + l * x_ls]
+ * (elements; interleaved complex when cplx). One of w_rs, w_ks, one of b_ks,
+ * b_ls and one of x_rs, x_ls should be 1 for coalesced access. */
+typedef struct icl_panel_apply_args {
+    int cplx, p, nlines;
+    const float* W;
+    long w_rs, w_ks;
+    const float* B;
+    long b_ks, b_ls;
+    float* X;
+    long x_rs, x_ls;
+} icl_panel_apply_args;
+hipError_t icl_panel_apply(const icl_panel_apply_args* a, hipStream_t stream);
+
 /* Inverses of the 256 x 256 diagonal panels of the canonical complex system
  * (panel p = unknowns [256 p, 256 p + plen), plen = min(256, nsys - 256 p)),
  * one workgroup per 64-wide column block of a panel, from the inverted 64 x 64
