@@ -237,11 +237,10 @@ namespace {
 
 constexpr int kLeaf = 64;
 // Blocked STRSM above 64 unknowns: the panel solve with inverted 256 x 256
-// diagonal panels (one GEMM per panel for the solve plus a GEMM update of the
-// rest) runs when no implementation is forced and kTrsmPanelDefault is set; the
-// panel solve with the in-workgroup panel kernel, the launch-per-64-block solve
-// and the single-launch dataflow solve are selectable by name.
-constexpr bool kTrsmPanelDefault = true;
+// diagonal panels (the panel's inverse applied to its right-hand sides, then a
+// GEMM update of the rest) runs unless another implementation is forced; the
+// panel solve with the in-workgroup panel kernel, the launch-per-64-block solve,
+// the single-launch dataflow solve and the recursive solve are selectable by name.
 constexpr int kPanel = 256;
 
 // X_p = W_p B_p (left: the panel's rows of B, all columns) or B_p W_p (right:
@@ -651,12 +650,10 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
         }
         const int nblk = (order + kLeaf - 1) / kLeaf;
         const size_t inv_elems = (size_t)nblk * kLeaf * kLeaf;
-        const bool dataflow = h->forced_impl == "strsm_dataflow64";
         const std::string& f = h->forced_impl;
-        const bool other_trsm = f == "strsm_panel256" || f == "strsm_blocked_inv64" || f == "strsm_dataflow64";
-        const bool panel_inv = f == "strsm_panel256_inv" || (kTrsmPanelDefault && !other_trsm);
-        const bool panel = panel_inv || h->forced_impl == "strsm_panel256" ||
-                           (kTrsmPanelDefault && h->forced_impl != "strsm_blocked_inv64" && !dataflow);
+        const bool dataflow = f == "strsm_dataflow64";
+        const bool panel = !dataflow && f != "strsm_blocked_inv64"; // the panel solves (the default among them:)
+        const bool panel_inv = panel && f != "strsm_panel256";       // with inverted panels
         if (panel) {
             // X in original order with B's shape (m x n, ld m); the canonical X view
             // mirrors B's. Panels of 256 unknowns are solved one after the other,
