@@ -541,18 +541,29 @@ sgemm_f32_kernel(GemmArgs g)
             }
             const float* as = As(cur);
             const float* bs = Bs(cur);
+            // Fragments of step kk + 2 are read into the second register set
+            // before step kk's products are issued, so each LDS read latency
+            // hides behind FM * FN MFMAs instead of stalling the wave.
+            float a[2][Cfg::FM], b[2][Cfg::FN];
+            auto fetch = [&](int kk, float(&fa)[Cfg::FM], float(&fb)[Cfg::FN]) {
+                const int kr = kk + (lane >> 5);
+#pragma unroll
+                for (int i = 0; i < Cfg::FM; ++i) fa[i] = as[kr * Cfg::SlabA::LD + wm0 + 32 * i + (lane & 31)];
+#pragma unroll
+                for (int j = 0; j < Cfg::FN; ++j) fb[j] = bs[kr * Cfg::SlabB::LD + wn0 + 32 * j + (lane & 31)];
+            };
+            fetch(0, a[0], b[0]);
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 2) {
-                const int kr = kk + (lane >> 5);
-                float a[Cfg::FM], b[Cfg::FN];
-#pragma unroll
-                for (int i = 0; i < Cfg::FM; ++i) a[i] = as[kr * Cfg::SlabA::LD + wm0 + 32 * i + (lane & 31)];
-#pragma unroll
-                for (int j = 0; j < Cfg::FN; ++j) b[j] = bs[kr * Cfg::SlabB::LD + wn0 + 32 * j + (lane & 31)];
+                const int s = (kk >> 1) & 1;
+                if (kk + 2 < BK) fetch(kk + 2, a[s ^ 1], b[s ^ 1]);
+                // Keeps the scheduler from sinking the reads back below the
+                // products (it would, to save the second register set).
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int i = 0; i < Cfg::FM; ++i)
 #pragma unroll
-                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x2_f32(b[j], a[i], acc[i][j]);
+                    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = mfma_32x32x2_f32(b[s][j], a[s][i], acc[i][j]);
             }
             if (kt + 1 < KT) {
                 sa.store(As(cur ^ 1), tid);
@@ -691,7 +702,7 @@ struct CgemmF32 {
 };
 
 template <int OA, int OB, int BM, int BN, int BK, int WAVES_M, int WAVES_N, bool EDGE, bool L3 = false>
-__global__ void __launch_bounds__(WAVES_M* WAVES_N * 64)
+__global__ void __launch_bounds__(WAVES_M* WAVES_N * 64, WAVES_M* WAVES_N <= 4 ? 2 : 1)
 cgemm_f32_kernel(GemmArgs g)
 {
     using Cfg = CgemmF32<OA, OB, BM, BN, BK, WAVES_M, WAVES_N, EDGE>;
@@ -748,36 +759,43 @@ cgemm_f32_kernel(GemmArgs g)
             const float* ai_s = Ai(cur);
             const float* br_s = Br(cur);
             const float* bi_s = Bi(cur);
-#pragma unroll
-            for (int kk = 0; kk < BK; kk += 2) {
+            // Fragments of step kk + 2 are read into the second register set
+            // before step kk's products are issued (as in the real kernel).
+            float ar[2][Cfg::FM], ai[2][Cfg::FM], br[2][Cfg::FN], bi[2][Cfg::FN];
+            auto fetch = [&](int kk, int s) {
                 const int kr = kk + (lane >> 5);
-                float ar[Cfg::FM], ai[Cfg::FM], br[Cfg::FN], bi[Cfg::FN], nbi[Cfg::FN];
 #pragma unroll
                 for (int i = 0; i < Cfg::FM; ++i) {
                     const int o = kr * Cfg::SlabA::LD + wm0 + 32 * i + (lane & 31);
-                    ar[i] = ar_s[o];
-                    ai[i] = ai_s[o];
+                    ar[s][i] = ar_s[o];
+                    ai[s][i] = ai_s[o];
                 }
 #pragma unroll
                 for (int j = 0; j < Cfg::FN; ++j) {
                     const int o = kr * Cfg::SlabB::LD + wn0 + 32 * j + (lane & 31);
-                    br[j] = br_s[o];
-                    bi[j] = bi_s[o];
-                    nbi[j] = -bi[j];
+                    br[s][j] = br_s[o];
+                    bi[s][j] = bi_s[o];
                 }
+            };
+            fetch(0, 0);
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 2) {
+                const int s = (kk >> 1) & 1;
+                if (kk + 2 < BK) fetch(kk + 2, s ^ 1);
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int i = 0; i < Cfg::FM; ++i)
 #pragma unroll
                     for (int j = 0; j < Cfg::FN; ++j) {
-                        accr[i][j] = mfma_32x32x2_f32(br[j], ar[i], accr[i][j]);
-                        acci[i][j] = mfma_32x32x2_f32(bi[j], ar[i], acci[i][j]);
+                        accr[i][j] = mfma_32x32x2_f32(br[s][j], ar[s][i], accr[i][j]);
+                        acci[i][j] = mfma_32x32x2_f32(bi[s][j], ar[s][i], acci[i][j]);
                     }
 #pragma unroll
                 for (int i = 0; i < Cfg::FM; ++i)
 #pragma unroll
                     for (int j = 0; j < Cfg::FN; ++j) {
-                        accr[i][j] = mfma_32x32x2_f32(nbi[j], ai[i], accr[i][j]);
-                        acci[i][j] = mfma_32x32x2_f32(br[j], ai[i], acci[i][j]);
+                        accr[i][j] = mfma_32x32x2_f32(-bi[s][j], ai[s][i], accr[i][j]);
+                        acci[i][j] = mfma_32x32x2_f32(br[s][j], ai[s][i], acci[i][j]);
                     }
             }
             if (kt + 1 < KT) {
