@@ -590,7 +590,7 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
 }
 
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
-               const float* A, long lda, float* B, long ldb, bool record)
+               const float* A, long lda, float* B, long ldb, bool record, bool reuse_inverses)
 {
     if (m <= 0 || n <= 0) return;
     if (alpha == 0.0f) {
@@ -681,8 +681,8 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
                 a.xv_si = t.fwd ? (long)m : -(long)m;
                 a.xv_sj = 1;
             }
-            hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
-            if (nfull) {
+            if (!reuse_inverses) hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
+            if (nfull && !reuse_inverses) {
                 // W_p (256 x 256, column-major) such that the panel's unknowns in
                 // original order are X_p = W_p B_p (left) or B_p W_p (right): the
                 // canonical inverse inv_c, written reversed in both indices for a
@@ -828,8 +828,10 @@ void strsm_refined_dev(iclblasContext* h, int side, int uplo, int trans, int dia
         r.ldr = m;
         r.alpha = (double)alpha;
         hip_check(icl_residual_f64(&r, h->stream), "residual_f64");
-        // D = op(A)^-1 R (in place in R), X += D
-        strsm_dev(h, side, uplo, trans, diag, m, n, 1.0f, A, lda, R, m, false);
+        // D = op(A)^-1 R (in place in R), X += D. The system is the first
+        // solve's, and nothing in between uses the TRSM workspace: its inverses
+        // are taken as they are.
+        strsm_dev(h, side, uplo, trans, diag, m, n, 1.0f, A, lda, R, m, false, true);
         hip_check(icl_axpby_2d(m, n, 1.0f, R, m, 1.0f, B, ldb, h->stream), "axpby");
         h->last_sweeps = s + 1;
         if (s + 1 == sweeps) break;

@@ -79,9 +79,13 @@ void syr2k_dev(iclblasContext* h, bool cplx, bool herm, int uplo, int trans, int
 void gemv_dev(iclblasContext* h, bool cplx, int trans, int m, int n, const float alpha[2], const float* A, long lda,
               const float* x, long incx, const float beta[2], float* y, long incy);
 
-// Blocked STRSM, in place on B.
+// Blocked STRSM, in place on B. reuse_inverses: the previous STRSM on this
+// handle was the same system (A, lda, side, uplo, trans, diag, m, n) and the
+// handle's TRSM workspace has not been used since, so the panel solve takes
+// the diagonal-block and panel inverses it left there instead of forming them
+// again (the other solves ignore the flag).
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
-               const float* A, long lda, float* B, long ldb, bool record = true);
+               const float* A, long lda, float* B, long ldb, bool record = true, bool reuse_inverses = false);
 
 // Blocked CTRSM on interleaved complex data, in place on B.
 void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, const float alpha[2],
