@@ -1,0 +1,175 @@
+// FP32 MFMA SGEMM fed by LDS DMA, 256 x 256 tiles, 8 waves (4 along m, 2 along n).
+//
+// Same product as sgemm_f32_kernel (reference iclBLAS.h:1648-1685; CPU definition
+// tests/blas_funcs/test_gemm.hpp:83-201), for problems without edges (M, N
+// multiples of 256, K of 16, 16-byte aligned operand columns); problems with
+// edges run sgemm_f32_kernel of the same tile.
+//
+// Each 16-deep slab of op(A) and op(B) is copied global -> LDS with
+// global_load_lds_dwordx4, one slab ahead of the MFMAs, so no slab passes through
+// VGPRs and no ds_write sits between the last MFMA of a slab and the barrier.
+// One DMA instruction writes 64 lanes x 16 bytes = 1 KB of contiguous LDS, in
+// one of two layouts by the operand's contiguous dimension:
+//  * rows contiguous (op(A) = A, op(B) = B^T): S[k][256], one instruction per k;
+//    a fragment read is one float per lane, 32 consecutive rows per half-wave.
+//  * k contiguous (op(A) = A^T, op(B) = B): S[row][16], 64-byte rows of four
+//    16-byte chunks, logical chunk c of row r stored in slot c ^ ((r >> 2) & 3);
+//    a 16-byte read serves 16 lanes from 256 bytes of banks, i.e. 4 rows, and
+//    over 16 consecutive rows at one logical chunk the (r & 3, slot) pairs are
+//    distinct, so the reads are free of bank conflicts.
+// A 16-byte read returns four k for one row, so within a slab the MFMAs take k
+// in the order k(j, h) = 4 * (2 * (j >> 2) + h) + (j & 3) for MFMA j = 0..7 and
+// lane half h (both operands alike: a common permutation of the summation index).
+#include "gemm_variants.hpp"
+
+namespace icl {
+
+typedef __attribute__((address_space(1))) void f32dma_global_void;
+typedef __attribute__((address_space(3))) void f32dma_lds_void;
+
+struct F32DmaCfg {
+    static constexpr int BM = 256, BN = 256, BK = 16, WVM = 4, WVN = 2;
+    static constexpr int NW = WVM * WVN, NT = NW * 64;
+    static constexpr int WM = BM / WVM, WN = BN / WVN;
+    static constexpr int FM = WM / 32, FN = WN / 32;
+    static constexpr int OPER_BYTES = 256 * BK * 4; // 16 KB per operand per stage
+    static constexpr int STAGE_BYTES = 2 * OPER_BYTES;
+    static constexpr size_t LDS_BYTES = 2 * (size_t)STAGE_BYTES;
+    static constexpr int INSTR = OPER_BYTES / 1024 / NW; // DMA instructions per wave per operand per slab
+    static_assert(INSTR * NW * 1024 == OPER_BYTES, "slab copies must divide among the waves");
+};
+
+__device__ __forceinline__ int f32dma_swizzle(int r) { return (r >> 2) & 3; }
+
+// Issue the DMA of one operand's slab (rows row0 .. row0 + 255, k0 .. k0 + 15).
+template <bool KC>
+__device__ __forceinline__ void f32dma_issue(const float* X, long ld, int row0, int k0, unsigned char* S, int wave,
+                                             int lane)
+{
+#pragma unroll
+    for (int i = 0; i < F32DmaCfg::INSTR; ++i) {
+        const int blk = wave + F32DmaCfg::NW * i; // 1 KB block of the slab, 0..15
+        const float* g;
+        if constexpr (!KC) {
+            // block = k row; lane covers rows 4 * lane .. 4 * lane + 3
+            g = X + (long)(k0 + blk) * ld + row0 + 4 * lane;
+        } else {
+            // block = rows 16 * blk .. + 15; lane -> (row, physical slot)
+            const int r = 16 * blk + (lane >> 2);
+            const int c = (lane & 3) ^ f32dma_swizzle(r);
+            g = X + (long)(row0 + r) * ld + k0 + 4 * c;
+        }
+        __builtin_amdgcn_global_load_lds((f32dma_global_void*)g, (f32dma_lds_void*)(S + blk * 1024), 16, 0, 0);
+    }
+}
+
+// Fragments of k-group q2 (MFMAs j = 4 * q2 .. 4 * q2 + 3) for F 32-row blocks
+// starting at row rb: frag[t][f] is row rb + 32 f + (lane & 31) at k = k(4 q2 + t, h).
+template <bool KC, int F>
+__device__ __forceinline__ void f32dma_frag(const float* S, int rb, int q2, int lane, float (&frag)[4][F])
+{
+    const int h = lane >> 5, q = 2 * q2 + h;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        const int r = rb + 32 * f + (lane & 31);
+        if constexpr (KC) {
+            const f32x4 v = *reinterpret_cast<const f32x4*>(S + r * 16 + ((q ^ f32dma_swizzle(r)) << 2));
+#pragma unroll
+            for (int t = 0; t < 4; ++t) frag[t][f] = v[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) frag[t][f] = S[(4 * q + t) * 256 + r];
+        }
+    }
+}
+
+template <bool A_KC, bool B_KC>
+__global__ void __launch_bounds__(F32DmaCfg::NT) sgemm_f32_dma_kernel(GemmArgs g)
+{
+    using Cfg = F32DmaCfg;
+    extern __shared__ __attribute__((aligned(1024))) unsigned char lds_f32dma[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm0 = (wave % Cfg::WVM) * Cfg::WM;
+    const int wn0 = (wave / Cfg::WVM) * Cfg::WN;
+    int tm, tn;
+    tile_coords<8>(g.M / Cfg::BM, g.N / Cfg::BN, tm, tn);
+    const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
+
+    f32x16 acc[Cfg::FM][Cfg::FN];
+#pragma unroll
+    for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = f32x16{};
+
+    auto issue = [&](int ks, int s) {
+        unsigned char* stage = lds_f32dma + s * Cfg::STAGE_BYTES;
+        f32dma_issue<A_KC>(g.A, g.lda, m0, ks * Cfg::BK, stage, wave, lane);
+        f32dma_issue<B_KC>(g.B, g.ldb, n0, ks * Cfg::BK, stage + Cfg::OPER_BYTES, wave, lane);
+    };
+
+    const int KS = g.K / Cfg::BK;
+    if (KS > 0) {
+        issue(0, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (int ks = 0; ks < KS; ++ks) {
+            const int cur = ks & 1;
+            if (ks + 1 < KS) issue(ks + 1, cur ^ 1);
+            const float* As = reinterpret_cast<const float*>(lds_f32dma + cur * Cfg::STAGE_BYTES);
+            const float* Bs = reinterpret_cast<const float*>(lds_f32dma + cur * Cfg::STAGE_BYTES + Cfg::OPER_BYTES);
+            float af[2][4][Cfg::FM], bf[2][4][Cfg::FN];
+            f32dma_frag<A_KC, Cfg::FM>(As, wm0, 0, lane, af[0]);
+            f32dma_frag<B_KC, Cfg::FN>(Bs, wn0, 0, lane, bf[0]);
+#pragma unroll
+            for (int q2 = 0; q2 < 2; ++q2) {
+                if (q2 == 0) {
+                    // second k-group's fragments read while the first group's MFMAs issue
+                    f32dma_frag<A_KC, Cfg::FM>(As, wm0, 1, lane, af[1]);
+                    f32dma_frag<B_KC, Cfg::FN>(Bs, wn0, 1, lane, bf[1]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+                        for (int j = 0; j < Cfg::FN; ++j)
+                            acc[i][j] = mfma_32x32x2_f32(bf[q2][t][j], af[q2][t][i], acc[i][j]);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < Cfg::FN; ++j)
+            store_real_tile<false, true>(g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
+}
+
+template <bool A_KC, bool B_KC>
+hipError_t run_sgemm_f32_dma(const GemmArgs* g, hipStream_t s)
+{
+    return launch_gemm<F32DmaCfg::BM, F32DmaCfg::BN, F32DmaCfg::NT, F32DmaCfg::LDS_BYTES>(
+        sgemm_f32_dma_kernel<A_KC, B_KC>, g, s);
+}
+
+// op(A) = A has rows (m) contiguous; op(B) = B has k contiguous.
+hipError_t sgemm_f32_dma_dispatch(const icl_gemm_args* g, int opa, int opb, int edge, void*, hipStream_t s)
+{
+    if (edge || uses_level3(g)) return sgemm_dispatch<false, 256, 256, 16, 4, 2>(g, opa, opb, edge, nullptr, s);
+    const bool a_kc = opa != 0, b_kc = opb == 0;
+    if (!a_kc && b_kc) return run_sgemm_f32_dma<false, true>(g, s);
+    if (!a_kc && !b_kc) return run_sgemm_f32_dma<false, false>(g, s);
+    if (a_kc && b_kc) return run_sgemm_f32_dma<true, true>(g, s);
+    return run_sgemm_f32_dma<true, false>(g, s);
+}
+
+} // namespace icl
+
+static const icl_gemm_variant kTable[] = {
+    {"sgemm_f32_dma_256x256w8", ICL_KIND_SGEMM, ICL_MATH_FP32, 256, 256, 16, 512, icl::sgemm_f32_dma_dispatch, nullptr,
+     141.0f, 1, 0},
+};
+ICL_REGISTER_VARIANTS(kTable)

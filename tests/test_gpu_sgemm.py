@@ -125,6 +125,31 @@ def test_sgemm_error_bound(h, mode, ta, tb, shape):
     assert h.last_implementation().startswith("sgemm_bf16x3" if split else "sgemm_f32")
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_sgemm_f32_dma_all_ops(h, ta, tb):
+    """The LDS-DMA FP32 kernel on a problem without edges (M, N multiples of
+    256, K of 16, leading dimensions multiples of 4), in every transpose case:
+    each operand is copied in the rows-contiguous or the k-contiguous layout by
+    its transpose, and the k order inside a slab is permuted for both alike."""
+    m, n, k = 512, 256, 272
+    ra, ca = (m, k) if ta == 0 else (k, m)
+    rb, cb = (k, n) if tb == 0 else (n, k)
+    A = datagen.matrix(ra, ca, ra + 4, "A", seed=24)
+    B = datagen.matrix(rb, cb, rb + 8, "B", seed=24)
+    C = datagen.matrix(m, n, m + 2, "C", seed=24)
+    C[m:] = -7.0
+    C0 = C.copy(order="F")
+    alpha, beta = 0.75, -0.5
+    C64, ab = oracle.dgemm(ta, tb, m, n, k, alpha, A, B, beta, C)
+    h.set_implementation("sgemm_f32_dma_256x256w8")
+    run(h, ta, tb, m, n, k, alpha, A, B, beta, C)
+    assert h.last_implementation() == "sgemm_f32_dma_256x256w8"
+    err = np.abs(C[:m].astype(np.float64) - C64)
+    extra = 2 * U32 * (abs(beta) * np.abs(C0[:m].astype(np.float64)) + abs(alpha) * ab)
+    assert bound_ratio(err, k, abs(alpha) * ab, extra) <= 4.0
+    assert np.all(C[m:] == -7.0)
+
+
 @pytest.mark.parametrize("impl", BF16_IMPLS)
 @pytest.mark.parametrize("shape", [(1, 1, 1), (33, 17, 16), (130, 70, 100)], ids=lambda s: "x".join(map(str, s)))
 def test_sgemm_split_forced_small_k_bound(h, impl, shape):
