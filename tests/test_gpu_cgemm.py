@@ -79,6 +79,30 @@ def test_cgemm_quarter_integer_grid_exact(h, impl, ta, tb, beta_zero):
     assert np.array_equal(C, ref, equal_nan=True)
 
 
+@pytest.mark.parametrize("ta", [0, 1, 2])
+@pytest.mark.parametrize("tb", [0, 1, 2])
+def test_cgemm_f32_dma_all_ops_exact(h, ta, tb):
+    """The LDS-DMA FP32 kernel on a problem without edges (M, N multiples of
+    128, K of 8, even leading dimensions), every op pair: each operand is copied
+    in the rows-contiguous or the k-contiguous layout by its op, op C negates
+    the imaginary fragment, and the k order inside a slab is permuted for both
+    operands alike. Quarter-integer data make every sum exact, so the result
+    equals cpu_gemm<complex<float>> bit for bit."""
+    m, n, k = 256, 128, 136
+    ra, ca = (m, k) if ta == 0 else (k, m)
+    rb, cb = (k, n) if tb == 0 else (n, k)
+    A = _qcmat(ra, ca, ra + 2, 0)
+    B = _qcmat(rb, cb, rb + 4, 1)
+    C = _qcmat(m, n, m + 1, 2)
+    alpha, beta = 0.75 - 0.5j, -0.25 + 1.0j
+    ref = C.copy(order="F")
+    oracle.cgemm(ta, tb, m, n, k, alpha, A, B, beta, ref)
+    h.set_implementation("cgemm_f32_dma_128x128")
+    run(h, ta, tb, m, n, k, alpha, A, B, beta, C)
+    assert h.last_implementation() == "cgemm_f32_dma_128x128"
+    assert np.array_equal(C, ref)
+
+
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("ta,tb", [(0, 0), (2, 0), (1, 2), (2, 2), (0, 1)])
 @pytest.mark.parametrize("shape", [(1, 1, 1), (37, 21, 53), (130, 70, 200)], ids=lambda s: "x".join(map(str, s)))
