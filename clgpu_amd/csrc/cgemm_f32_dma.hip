@@ -90,7 +90,10 @@ __device__ __forceinline__ void c32dma_frag(const float* S, int rb, int q2, int 
     }
 }
 
-template <bool A_KC, bool CA, bool B_KC, bool CB>
+// L3: the level-3 instantiation -- tiles of the written triangle only (write_tri),
+// the slabs of the tile's k window only (kband), and the level-3 epilogue
+// (triangle stores, conjugated accumulator, real diagonal).
+template <bool A_KC, bool CA, bool B_KC, bool CB, bool L3>
 __global__ void __launch_bounds__(C32DmaCfg::NT, 2) cgemm_f32_dma_kernel(GemmArgs g)
 {
     using Cfg = C32DmaCfg;
@@ -100,8 +103,18 @@ __global__ void __launch_bounds__(C32DmaCfg::NT, 2) cgemm_f32_dma_kernel(GemmArg
     const int wm0 = (wave % Cfg::WVM) * Cfg::WM;
     const int wn0 = (wave / Cfg::WVM) * Cfg::WN;
     int tm, tn;
-    tile_coords<8>(g.M / Cfg::BM, g.N / Cfg::BN, tm, tn);
+    if constexpr (L3) {
+        if (!tile_coords_level3<Cfg::BM, Cfg::BN>(g.write_tri, g.kband, g.M / Cfg::BM, g.N / Cfg::BN, tm, tn)) return;
+    } else
+        tile_coords<8>(g.M / Cfg::BM, g.N / Cfg::BN, tm, tn);
     const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
+    int ks0 = 0, ks1 = g.K / Cfg::BK; // slabs [ks0, ks1) of the tile's k window
+    if constexpr (L3) {
+        int kb = 0, ke = 0;
+        if (!tile_window<Cfg::BM, Cfg::BN>(g, m0, n0, kb, ke)) return;
+        ks0 = kb / Cfg::BK;
+        ks1 = ke > kb ? (ke + Cfg::BK - 1) / Cfg::BK : ks0;
+    }
 
     f32x16 accr[Cfg::FM][Cfg::FN], acci[Cfg::FM][Cfg::FN];
 #pragma unroll
@@ -118,14 +131,13 @@ __global__ void __launch_bounds__(C32DmaCfg::NT, 2) cgemm_f32_dma_kernel(GemmArg
         c32dma_issue<B_KC>(g.B, g.ldb, n0, ks * Cfg::BK, stage + Cfg::OPER_BYTES, wave, lane);
     };
 
-    const int KS = g.K / Cfg::BK;
-    if (KS > 0) {
-        issue(0, 0);
+    if (ks0 < ks1) {
+        issue(ks0, 0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        for (int ks = 0; ks < KS; ++ks) {
-            const int cur = ks & 1;
-            if (ks + 1 < KS) issue(ks + 1, cur ^ 1);
+        for (int ks = ks0; ks < ks1; ++ks) {
+            const int cur = (ks - ks0) & 1;
+            if (ks + 1 < ks1) issue(ks + 1, cur ^ 1);
             const float* As = reinterpret_cast<const float*>(lds_c32dma + cur * Cfg::STAGE_BYTES);
             const float* Bs = reinterpret_cast<const float*>(lds_c32dma + cur * Cfg::STAGE_BYTES + Cfg::OPER_BYTES);
             float ar[2][2][Cfg::FM], ai[2][2][Cfg::FM], br[2][2][Cfg::FN], bi[2][2][Cfg::FN];
@@ -165,21 +177,24 @@ __global__ void __launch_bounds__(C32DmaCfg::NT, 2) cgemm_f32_dma_kernel(GemmArg
     for (int i = 0; i < Cfg::FM; ++i)
 #pragma unroll
         for (int j = 0; j < Cfg::FN; ++j)
-            store_complex_tile<false>(g, accr[i][j], acci[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j,
-                                      lane);
+            store_complex_tile<false, L3>(g, accr[i][j], acci[i][j], m0 + wm0 + 32 * i + (lane & 31),
+                                          n0 + wn0 + 32 * j, lane);
 }
 
 template <bool A_KC, bool CA, bool B_KC, bool CB>
 hipError_t run_cgemm_f32_dma(const GemmArgs* g, hipStream_t s)
 {
+    if (uses_level3(g))
+        return launch_gemm<C32DmaCfg::BM, C32DmaCfg::BN, C32DmaCfg::NT, C32DmaCfg::LDS_BYTES>(
+            cgemm_f32_dma_kernel<A_KC, CA, B_KC, CB, true>, g, s);
     return launch_gemm<C32DmaCfg::BM, C32DmaCfg::BN, C32DmaCfg::NT, C32DmaCfg::LDS_BYTES>(
-        cgemm_f32_dma_kernel<A_KC, CA, B_KC, CB>, g, s);
+        cgemm_f32_dma_kernel<A_KC, CA, B_KC, CB, false>, g, s);
 }
 
 // op values: 0 = N, 1 = T, 2 = C. op(A) = A has rows (m) contiguous; op(B) = B has k contiguous.
 hipError_t cgemm_f32_dma_dispatch(const icl_gemm_args* g, int opa, int opb, int edge, void* ws, hipStream_t s)
 {
-    if (edge || uses_level3(g)) return cgemm_dispatch<false, 128, 128, 8, 2, 2>(g, opa, opb, edge, ws, s);
+    if (edge) return cgemm_dispatch<false, 128, 128, 8, 2, 2>(g, opa, opb, edge, ws, s);
     const int oa = opa < 0 || opa > 2 ? 1 : opa;
     const int ob = opb < 0 || opb > 2 ? 1 : opb;
     switch (oa * 3 + ob) {
@@ -199,6 +214,6 @@ hipError_t cgemm_f32_dma_dispatch(const icl_gemm_args* g, int opa, int opb, int 
 
 static const icl_gemm_variant kTable[] = {
     {"cgemm_f32_dma_128x128", ICL_KIND_CGEMM, ICL_MATH_FP32, 128, 128, 8, 256, icl::cgemm_f32_dma_dispatch, nullptr,
-     142.5f, 2, 0},
+     142.5f, 2, 1},
 };
 ICL_REGISTER_VARIANTS(kTable)

@@ -83,7 +83,10 @@ __device__ __forceinline__ void f32dma_frag(const float* S, int rb, int q2, int 
     }
 }
 
-template <bool A_KC, bool B_KC>
+// L3: the level-3 instantiation -- tiles of the written triangle only (write_tri),
+// the slabs of the tile's k window only (kband); the epilogue always honours
+// write_tri. The plain instantiation is compiled without the first two.
+template <bool A_KC, bool B_KC, bool L3>
 __global__ void __launch_bounds__(F32DmaCfg::NT) sgemm_f32_dma_kernel(GemmArgs g)
 {
     using Cfg = F32DmaCfg;
@@ -93,8 +96,18 @@ __global__ void __launch_bounds__(F32DmaCfg::NT) sgemm_f32_dma_kernel(GemmArgs g
     const int wm0 = (wave % Cfg::WVM) * Cfg::WM;
     const int wn0 = (wave / Cfg::WVM) * Cfg::WN;
     int tm, tn;
-    tile_coords<8>(g.M / Cfg::BM, g.N / Cfg::BN, tm, tn);
+    if constexpr (L3) {
+        if (!tile_coords_level3<Cfg::BM, Cfg::BN>(g.write_tri, g.kband, g.M / Cfg::BM, g.N / Cfg::BN, tm, tn)) return;
+    } else
+        tile_coords<8>(g.M / Cfg::BM, g.N / Cfg::BN, tm, tn);
     const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
+    int ks0 = 0, ks1 = g.K / Cfg::BK; // slabs [ks0, ks1) of the tile's k window
+    if constexpr (L3) {
+        int kb = 0, ke = 0;
+        if (!tile_window<Cfg::BM, Cfg::BN>(g, m0, n0, kb, ke)) return;
+        ks0 = kb / Cfg::BK;
+        ks1 = ke > kb ? (ke + Cfg::BK - 1) / Cfg::BK : ks0;
+    }
 
     f32x16 acc[Cfg::FM][Cfg::FN];
 #pragma unroll
@@ -108,14 +121,13 @@ __global__ void __launch_bounds__(F32DmaCfg::NT) sgemm_f32_dma_kernel(GemmArgs g
         f32dma_issue<B_KC>(g.B, g.ldb, n0, ks * Cfg::BK, stage + Cfg::OPER_BYTES, wave, lane);
     };
 
-    const int KS = g.K / Cfg::BK;
-    if (KS > 0) {
-        issue(0, 0);
+    if (ks0 < ks1) {
+        issue(ks0, 0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        for (int ks = 0; ks < KS; ++ks) {
-            const int cur = ks & 1;
-            if (ks + 1 < KS) issue(ks + 1, cur ^ 1);
+        for (int ks = ks0; ks < ks1; ++ks) {
+            const int cur = (ks - ks0) & 1;
+            if (ks + 1 < ks1) issue(ks + 1, cur ^ 1);
             const float* As = reinterpret_cast<const float*>(lds_f32dma + cur * Cfg::STAGE_BYTES);
             const float* Bs = reinterpret_cast<const float*>(lds_f32dma + cur * Cfg::STAGE_BYTES + Cfg::OPER_BYTES);
             float af[2][4][Cfg::FM], bf[2][4][Cfg::FN];
@@ -151,14 +163,17 @@ __global__ void __launch_bounds__(F32DmaCfg::NT) sgemm_f32_dma_kernel(GemmArgs g
 template <bool A_KC, bool B_KC>
 hipError_t run_sgemm_f32_dma(const GemmArgs* g, hipStream_t s)
 {
+    if (uses_level3(g))
+        return launch_gemm<F32DmaCfg::BM, F32DmaCfg::BN, F32DmaCfg::NT, F32DmaCfg::LDS_BYTES>(
+            sgemm_f32_dma_kernel<A_KC, B_KC, true>, g, s);
     return launch_gemm<F32DmaCfg::BM, F32DmaCfg::BN, F32DmaCfg::NT, F32DmaCfg::LDS_BYTES>(
-        sgemm_f32_dma_kernel<A_KC, B_KC>, g, s);
+        sgemm_f32_dma_kernel<A_KC, B_KC, false>, g, s);
 }
 
 // op(A) = A has rows (m) contiguous; op(B) = B has k contiguous.
 hipError_t sgemm_f32_dma_dispatch(const icl_gemm_args* g, int opa, int opb, int edge, void*, hipStream_t s)
 {
-    if (edge || uses_level3(g)) return sgemm_dispatch<false, 256, 256, 16, 4, 2>(g, opa, opb, edge, nullptr, s);
+    if (edge) return sgemm_dispatch<false, 256, 256, 16, 4, 2>(g, opa, opb, edge, nullptr, s);
     const bool a_kc = opa != 0, b_kc = opb == 0;
     if (!a_kc && b_kc) return run_sgemm_f32_dma<false, true>(g, s);
     if (!a_kc && !b_kc) return run_sgemm_f32_dma<false, false>(g, s);
@@ -170,6 +185,6 @@ hipError_t sgemm_f32_dma_dispatch(const icl_gemm_args* g, int opa, int opb, int 
 
 static const icl_gemm_variant kTable[] = {
     {"sgemm_f32_dma_256x256w8", ICL_KIND_SGEMM, ICL_MATH_FP32, 256, 256, 16, 512, icl::sgemm_f32_dma_dispatch, nullptr,
-     141.0f, 1, 0},
+     141.0f, 1, 1},
 };
 ICL_REGISTER_VARIANTS(kTable)

@@ -119,6 +119,40 @@ def test_level3_variants_vs_oracle(h, routine, v, shape):
     assert h.last_implementation().startswith(("sgemm_f32", "cgemm_f32"))
 
 
+DMA_L3 = [("Ssyrk", dict(uplo=u, trans=t), (0, 512, 272)) for u in (0, 1) for t in (0, 1)] + [
+    ("Ssyr2k", dict(uplo=1, trans=1), (0, 512, 256)),
+    ("Strmm", dict(side=0, uplo=0, trans=1, diag=0), (512, 256, 0)),
+    ("Strmm", dict(side=1, uplo=1, trans=0, diag=1), (256, 512, 0)),
+    ("Cherk", dict(uplo=0, trans=2), (0, 256, 136)),
+    ("Cherk", dict(uplo=1, trans=0), (0, 256, 136)),
+    ("Csyrk", dict(uplo=1, trans=1), (0, 256, 128)),
+    ("Cher2k", dict(uplo=0, trans=0), (0, 256, 128)),
+    ("Ctrmm", dict(side=1, uplo=1, trans=2, diag=1), (256, 384, 0)),
+    ("Ctrmm", dict(side=0, uplo=0, trans=0, diag=0), (384, 256, 0))]
+
+
+@pytest.mark.parametrize("routine,v,shape", DMA_L3, ids=_vid)
+def test_level3_dma_kernels_forced(h, routine, v, shape):
+    """The LDS-DMA FP32 GEMM kernels forced on level-3 problems without edges:
+    the written triangle's tiles only (SYRK / HERK types), the k windows of a
+    triangular operand (TRMM), the conjugated accumulator and real diagonal of
+    the complex epilogue."""
+    m, n, k = shape
+    impl = "cgemm_f32_dma_128x128" if routine in COMPLEX else "sgemm_f32_dma_256x256w8"
+    rng = np.random.default_rng(m + 7 * n + 13 * k)
+    c = random_case(routine, rng, m=m or None, n=n, k=k or None, pad=0, **v)
+    h.set_implementation(impl)
+    got = _run(h, c)
+    exp, mag, kk = oracle_result(c, c["A"], c["B"], c["C"])
+    (rc, cc) = shapes(c)[2]
+    ok, ratio = _bound_ok(c, got, exp, mag, kk)
+    assert ok, ratio
+    if routine in RANKK or routine in RANK2K:
+        other = np.triu(np.ones((rc, cc), bool), 1) if v["uplo"] == 1 else np.tril(np.ones((rc, cc), bool), -1)
+        assert np.array_equal(got[:rc, :cc][other], c["C"][:rc, :cc][other])
+    assert h.last_implementation() == impl
+
+
 @pytest.mark.parametrize("routine,v", ALL, ids=_vid)
 def test_level3_split_bf16_vs_oracle(h, routine, v):
     """Split-BF16 math mode with every product at least 256 deep: the split
