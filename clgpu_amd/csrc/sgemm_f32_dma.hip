@@ -1,8 +1,9 @@
-// FP32 MFMA SGEMM fed by LDS DMA, 256 x 256 tiles, 8 waves (4 along m, 2 along n).
+// FP32 MFMA SGEMM fed by LDS DMA: 256 x 256 tiles with 8 waves (4 along m, 2 along n)
+// and 128 x 128 tiles with 4 waves (2 x 2).
 //
 // Same product as sgemm_f32_kernel (reference iclBLAS.h:1648-1685; CPU definition
 // tests/blas_funcs/test_gemm.hpp:83-201), for problems without edges (M, N
-// multiples of 256, K of 16, 16-byte aligned operand columns); problems with
+// multiples of the tile, K of 16, 16-byte aligned operand columns); problems with
 // edges run sgemm_f32_kernel of the same tile.
 //
 // Each 16-deep slab of op(A) and op(B) is copied global -> LDS with
@@ -10,7 +11,8 @@
 // VGPRs and no ds_write sits between the last MFMA of a slab and the barrier.
 // One DMA instruction writes 64 lanes x 16 bytes = 1 KB of contiguous LDS, in
 // one of two layouts by the operand's contiguous dimension:
-//  * rows contiguous (op(A) = A, op(B) = B^T): S[k][256], one instruction per k;
+//  * rows contiguous (op(A) = A, op(B) = B^T): S[k][rows], one instruction per k
+//    row of a 256-row tile, per two k rows of a 128-row tile;
 //    a fragment read is one float per lane, 32 consecutive rows per half-wave.
 //  * k contiguous (op(A) = A^T, op(B) = B): S[row][16], 64-byte rows of four
 //    16-byte chunks, logical chunk c of row r stored in slot c ^ ((r >> 2) & 3);
@@ -27,32 +29,36 @@ namespace icl {
 typedef __attribute__((address_space(1))) void f32dma_global_void;
 typedef __attribute__((address_space(3))) void f32dma_lds_void;
 
+template <int BM_, int BN_, int WVM_, int WVN_>
 struct F32DmaCfg {
-    static constexpr int BM = 256, BN = 256, BK = 16, WVM = 4, WVN = 2;
+    static constexpr int BM = BM_, BN = BN_, BK = 16, WVM = WVM_, WVN = WVN_;
     static constexpr int NW = WVM * WVN, NT = NW * 64;
     static constexpr int WM = BM / WVM, WN = BN / WVN;
     static constexpr int FM = WM / 32, FN = WN / 32;
-    static constexpr int OPER_BYTES = 256 * BK * 4; // 16 KB per operand per stage
-    static constexpr int STAGE_BYTES = 2 * OPER_BYTES;
+    static constexpr int A_BYTES = BM * BK * 4, B_BYTES = BN * BK * 4; // per stage
+    static constexpr int STAGE_BYTES = A_BYTES + B_BYTES;
     static constexpr size_t LDS_BYTES = 2 * (size_t)STAGE_BYTES;
-    static constexpr int INSTR = OPER_BYTES / 1024 / NW; // DMA instructions per wave per operand per slab
-    static_assert(INSTR * NW * 1024 == OPER_BYTES, "slab copies must divide among the waves");
+    static_assert(A_BYTES % (1024 * NW) == 0 && B_BYTES % (1024 * NW) == 0, "slab copies must divide among the waves");
+    static_assert(BM == 128 || BM == 256, "a k row of a rows-contiguous slab fills 1 KB or half of it");
+    static_assert(BN == 128 || BN == 256, "a k row of a rows-contiguous slab fills 1 KB or half of it");
 };
 
 __device__ __forceinline__ int f32dma_swizzle(int r) { return (r >> 2) & 3; }
 
-// Issue the DMA of one operand's slab (rows row0 .. row0 + 255, k0 .. k0 + 15).
-template <bool KC>
+// Issue the DMA of one operand's slab (rows row0 .. row0 + ROWS - 1, k0 .. k0 + 15).
+template <bool KC, int ROWS, int NW>
 __device__ __forceinline__ void f32dma_issue(const float* X, long ld, int row0, int k0, unsigned char* S, int wave,
                                              int lane)
 {
+    constexpr int KR = 256 / ROWS; // k rows per 1 KB block (rows-contiguous layout)
 #pragma unroll
-    for (int i = 0; i < F32DmaCfg::INSTR; ++i) {
-        const int blk = wave + F32DmaCfg::NW * i; // 1 KB block of the slab, 0..15
+    for (int i = 0; i < ROWS * 16 * 4 / 1024 / NW; ++i) {
+        const int blk = wave + NW * i; // 1 KB block of the slab
         const float* g;
         if constexpr (!KC) {
-            // block = k row; lane covers rows 4 * lane .. 4 * lane + 3
-            g = X + (long)(k0 + blk) * ld + row0 + 4 * lane;
+            // block = k rows KR * blk ..; lane covers 4 consecutive rows of one of them
+            const int kr = KR * blk + lane / (64 / KR);
+            g = X + (long)(k0 + kr) * ld + row0 + 4 * (lane % (64 / KR));
         } else {
             // block = rows 16 * blk .. + 15; lane -> (row, physical slot)
             const int r = 16 * blk + (lane >> 2);
@@ -65,7 +71,7 @@ __device__ __forceinline__ void f32dma_issue(const float* X, long ld, int row0, 
 
 // Fragments of k-group q2 (MFMAs j = 4 * q2 .. 4 * q2 + 3) for F 32-row blocks
 // starting at row rb: frag[t][f] is row rb + 32 f + (lane & 31) at k = k(4 q2 + t, h).
-template <bool KC, int F>
+template <bool KC, int F, int ROWS>
 __device__ __forceinline__ void f32dma_frag(const float* S, int rb, int q2, int lane, float (&frag)[4][F])
 {
     const int h = lane >> 5, q = 2 * q2 + h;
@@ -78,7 +84,7 @@ __device__ __forceinline__ void f32dma_frag(const float* S, int rb, int q2, int 
             for (int t = 0; t < 4; ++t) frag[t][f] = v[t];
         } else {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) frag[t][f] = S[(4 * q + t) * 256 + r];
+            for (int t = 0; t < 4; ++t) frag[t][f] = S[(4 * q + t) * ROWS + r];
         }
     }
 }
@@ -86,10 +92,10 @@ __device__ __forceinline__ void f32dma_frag(const float* S, int rb, int q2, int 
 // L3: the level-3 instantiation -- tiles of the written triangle only (write_tri),
 // the slabs of the tile's k window only (kband); the epilogue always honours
 // write_tri. The plain instantiation is compiled without the first two.
-template <bool A_KC, bool B_KC, bool L3>
-__global__ void __launch_bounds__(F32DmaCfg::NT) sgemm_f32_dma_kernel(GemmArgs g)
+template <int BM, int BN, int WVM, int WVN, bool A_KC, bool B_KC, bool L3>
+__global__ void __launch_bounds__(WVM* WVN * 64) sgemm_f32_dma_kernel(GemmArgs g)
 {
-    using Cfg = F32DmaCfg;
+    using Cfg = F32DmaCfg<BM, BN, WVM, WVN>;
     extern __shared__ __attribute__((aligned(1024))) unsigned char lds_f32dma[];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -117,8 +123,8 @@ __global__ void __launch_bounds__(F32DmaCfg::NT) sgemm_f32_dma_kernel(GemmArgs g
 
     auto issue = [&](int ks, int s) {
         unsigned char* stage = lds_f32dma + s * Cfg::STAGE_BYTES;
-        f32dma_issue<A_KC>(g.A, g.lda, m0, ks * Cfg::BK, stage, wave, lane);
-        f32dma_issue<B_KC>(g.B, g.ldb, n0, ks * Cfg::BK, stage + Cfg::OPER_BYTES, wave, lane);
+        f32dma_issue<A_KC, BM, Cfg::NW>(g.A, g.lda, m0, ks * Cfg::BK, stage, wave, lane);
+        f32dma_issue<B_KC, BN, Cfg::NW>(g.B, g.ldb, n0, ks * Cfg::BK, stage + Cfg::A_BYTES, wave, lane);
     };
 
     if (ks0 < ks1) {
@@ -129,16 +135,16 @@ __global__ void __launch_bounds__(F32DmaCfg::NT) sgemm_f32_dma_kernel(GemmArgs g
             const int cur = (ks - ks0) & 1;
             if (ks + 1 < ks1) issue(ks + 1, cur ^ 1);
             const float* As = reinterpret_cast<const float*>(lds_f32dma + cur * Cfg::STAGE_BYTES);
-            const float* Bs = reinterpret_cast<const float*>(lds_f32dma + cur * Cfg::STAGE_BYTES + Cfg::OPER_BYTES);
+            const float* Bs = reinterpret_cast<const float*>(lds_f32dma + cur * Cfg::STAGE_BYTES + Cfg::A_BYTES);
             float af[2][4][Cfg::FM], bf[2][4][Cfg::FN];
-            f32dma_frag<A_KC, Cfg::FM>(As, wm0, 0, lane, af[0]);
-            f32dma_frag<B_KC, Cfg::FN>(Bs, wn0, 0, lane, bf[0]);
+            f32dma_frag<A_KC, Cfg::FM, BM>(As, wm0, 0, lane, af[0]);
+            f32dma_frag<B_KC, Cfg::FN, BN>(Bs, wn0, 0, lane, bf[0]);
 #pragma unroll
             for (int q2 = 0; q2 < 2; ++q2) {
                 if (q2 == 0) {
                     // second k-group's fragments read while the first group's MFMAs issue
-                    f32dma_frag<A_KC, Cfg::FM>(As, wm0, 1, lane, af[1]);
-                    f32dma_frag<B_KC, Cfg::FN>(Bs, wn0, 1, lane, bf[1]);
+                    f32dma_frag<A_KC, Cfg::FM, BM>(As, wm0, 1, lane, af[1]);
+                    f32dma_frag<B_KC, Cfg::FN, BN>(Bs, wn0, 1, lane, bf[1]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -160,31 +166,39 @@ __global__ void __launch_bounds__(F32DmaCfg::NT) sgemm_f32_dma_kernel(GemmArgs g
             store_real_tile<false, true>(g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
 }
 
-template <bool A_KC, bool B_KC>
+template <int BM, int BN, int WVM, int WVN, bool A_KC, bool B_KC>
 hipError_t run_sgemm_f32_dma(const GemmArgs* g, hipStream_t s)
 {
+    using Cfg = F32DmaCfg<BM, BN, WVM, WVN>;
     if (uses_level3(g))
-        return launch_gemm<F32DmaCfg::BM, F32DmaCfg::BN, F32DmaCfg::NT, F32DmaCfg::LDS_BYTES>(
-            sgemm_f32_dma_kernel<A_KC, B_KC, true>, g, s);
-    return launch_gemm<F32DmaCfg::BM, F32DmaCfg::BN, F32DmaCfg::NT, F32DmaCfg::LDS_BYTES>(
-        sgemm_f32_dma_kernel<A_KC, B_KC, false>, g, s);
+        return launch_gemm<BM, BN, Cfg::NT, Cfg::LDS_BYTES>(sgemm_f32_dma_kernel<BM, BN, WVM, WVN, A_KC, B_KC, true>,
+                                                            g, s);
+    return launch_gemm<BM, BN, Cfg::NT, Cfg::LDS_BYTES>(sgemm_f32_dma_kernel<BM, BN, WVM, WVN, A_KC, B_KC, false>, g,
+                                                        s);
 }
 
-// op(A) = A has rows (m) contiguous; op(B) = B has k contiguous.
+// op(A) = A has rows (m) contiguous; op(B) = B has k contiguous. Problems with
+// edges run sgemm_f32_kernel of the same tile and wave layout, and so does op
+// N x T at 128 x 128, where both operands are rows-contiguous and that kernel
+// measured faster (8192^3: 134.9 against 132.3 TFLOP/s).
+template <int BM, int BN, int WVM, int WVN>
 hipError_t sgemm_f32_dma_dispatch(const icl_gemm_args* g, int opa, int opb, int edge, void*, hipStream_t s)
 {
-    if (edge) return sgemm_dispatch<false, 256, 256, 16, 4, 2>(g, opa, opb, edge, nullptr, s);
     const bool a_kc = opa != 0, b_kc = opb == 0;
-    if (!a_kc && b_kc) return run_sgemm_f32_dma<false, true>(g, s);
-    if (!a_kc && !b_kc) return run_sgemm_f32_dma<false, false>(g, s);
-    if (a_kc && b_kc) return run_sgemm_f32_dma<true, true>(g, s);
-    return run_sgemm_f32_dma<true, false>(g, s);
+    if (edge || (BM == 128 && !a_kc && !b_kc))
+        return sgemm_dispatch<false, BM, BN, 16, WVM, WVN>(g, opa, opb, edge, nullptr, s);
+    if (!a_kc && b_kc) return run_sgemm_f32_dma<BM, BN, WVM, WVN, false, true>(g, s);
+    if (!a_kc && !b_kc) return run_sgemm_f32_dma<BM, BN, WVM, WVN, false, false>(g, s);
+    if (a_kc && b_kc) return run_sgemm_f32_dma<BM, BN, WVM, WVN, true, true>(g, s);
+    return run_sgemm_f32_dma<BM, BN, WVM, WVN, true, false>(g, s);
 }
 
 } // namespace icl
 
 static const icl_gemm_variant kTable[] = {
-    {"sgemm_f32_dma_256x256w8", ICL_KIND_SGEMM, ICL_MATH_FP32, 256, 256, 16, 512, icl::sgemm_f32_dma_dispatch, nullptr,
-     141.0f, 1, 1},
+    {"sgemm_f32_dma_256x256w8", ICL_KIND_SGEMM, ICL_MATH_FP32, 256, 256, 16, 512,
+     icl::sgemm_f32_dma_dispatch<256, 256, 4, 2>, nullptr, 141.0f, 1, 1},
+    {"sgemm_f32_dma_128x128", ICL_KIND_SGEMM, ICL_MATH_FP32, 128, 128, 16, 256,
+     icl::sgemm_f32_dma_dispatch<128, 128, 2, 2>, nullptr, 136.5f, 2, 1},
 };
 ICL_REGISTER_VARIANTS(kTable)

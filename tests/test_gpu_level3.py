@@ -128,17 +128,21 @@ DMA_L3 = [("Ssyrk", dict(uplo=u, trans=t), (0, 512, 272)) for u in (0, 1) for t 
     ("Csyrk", dict(uplo=1, trans=1), (0, 256, 128)),
     ("Cher2k", dict(uplo=0, trans=0), (0, 256, 128)),
     ("Ctrmm", dict(side=1, uplo=1, trans=2, diag=1), (256, 384, 0)),
-    ("Ctrmm", dict(side=0, uplo=0, trans=0, diag=0), (384, 256, 0))]
+    ("Ctrmm", dict(side=0, uplo=0, trans=0, diag=0), (384, 256, 0)),
+    ("Ssyrk", dict(uplo=0, trans=1), (0, 384, 144), "sgemm_f32_dma_128x128"),
+    ("Ssyr2k", dict(uplo=1, trans=0), (0, 384, 128), "sgemm_f32_dma_128x128"),
+    ("Strmm", dict(side=1, uplo=0, trans=1, diag=0), (128, 384, 0), "sgemm_f32_dma_128x128")]
 
 
-@pytest.mark.parametrize("routine,v,shape", DMA_L3, ids=_vid)
-def test_level3_dma_kernels_forced(h, routine, v, shape):
+@pytest.mark.parametrize("routine,v,shape,impl", [
+    c if len(c) == 4 else c + ("cgemm_f32_dma_128x128" if c[0] in COMPLEX else "sgemm_f32_dma_256x256w8",)
+    for c in DMA_L3], ids=_vid)
+def test_level3_dma_kernels_forced(h, routine, v, shape, impl):
     """The LDS-DMA FP32 GEMM kernels forced on level-3 problems without edges:
     the written triangle's tiles only (SYRK / HERK types), the k windows of a
     triangular operand (TRMM), the conjugated accumulator and real diagonal of
     the complex epilogue."""
     m, n, k = shape
-    impl = "cgemm_f32_dma_128x128" if routine in COMPLEX else "sgemm_f32_dma_256x256w8"
     rng = np.random.default_rng(m + 7 * n + 13 * k)
     c = random_case(routine, rng, m=m or None, n=n, k=k or None, pad=0, **v)
     h.set_implementation(impl)

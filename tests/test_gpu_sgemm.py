@@ -125,8 +125,9 @@ def test_sgemm_error_bound(h, mode, ta, tb, shape):
     assert h.last_implementation().startswith("sgemm_bf16x3" if split else "sgemm_f32")
 
 
+@pytest.mark.parametrize("impl", ["sgemm_f32_dma_256x256w8", "sgemm_f32_dma_128x128"])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-def test_sgemm_f32_dma_all_ops(h, ta, tb):
+def test_sgemm_f32_dma_all_ops(h, ta, tb, impl):
     """The LDS-DMA FP32 kernel on a problem without edges (M, N multiples of
     256, K of 16, leading dimensions multiples of 4), in every transpose case:
     each operand is copied in the rows-contiguous or the k-contiguous layout by
@@ -141,9 +142,9 @@ def test_sgemm_f32_dma_all_ops(h, ta, tb):
     C0 = C.copy(order="F")
     alpha, beta = 0.75, -0.5
     C64, ab = oracle.dgemm(ta, tb, m, n, k, alpha, A, B, beta, C)
-    h.set_implementation("sgemm_f32_dma_256x256w8")
+    h.set_implementation(impl)
     run(h, ta, tb, m, n, k, alpha, A, B, beta, C)
-    assert h.last_implementation() == "sgemm_f32_dma_256x256w8"
+    assert h.last_implementation() == impl
     err = np.abs(C[:m].astype(np.float64) - C64)
     extra = 2 * U32 * (abs(beta) * np.abs(C0[:m].astype(np.float64)) + abs(alpha) * ab)
     assert bound_ratio(err, k, abs(alpha) * ab, extra) <= 4.0
