@@ -55,28 +55,85 @@ def parse():
     return ap.parse_args()
 
 
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_threads() -> int:
+    """Threads for the all-core leg: the CPUs this process may run on, capped by
+    OMP_NUM_THREADS where the host sets it (the CPU share granted to this job)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def _slab_rate(run, m: int, n: int, k: int, step: int, target_s: float):
+    """Times run(cols) on growing column slabs until one takes target_s; returns
+    (TFLOP/s, cols, seconds)."""
+    cols = step
+    while True:
+        dt = run(cols)
+        if dt >= target_s or cols >= n:
+            break
+        cols = min(n, max(cols + step, int(cols * min(8.0, 1.2 * target_s / max(dt, 1e-3)))))
+        cols = max(step, (cols // step) * step)
+    return 2.0 * m * cols * k / dt / 1e12, cols, dt
+
+
 def cpu_baseline(A: np.ndarray, B: np.ndarray, n: int, target_s: float) -> dict:
-    """Times the restated reference CPU GEMM (oracle.sgemm_mt = cpu_gemm split over
-    output columns) on a slab of columns of the same workload."""
+    """The reference's CPU GEMM on the host cores, on column slabs of the same
+    workload, extrapolated linearly in n:
+      - all cores: the restated cpu_gemm (oracle.sgemm_mt: the reference's loop
+        order and double accumulator, output columns split over threads);
+      - one thread: the reference's own cpu_gemm (tests/blas_funcs/test_gemm.hpp:83-201,
+        compiled from its headers into oracle/_ref) where that build is present,
+        else the restatement on one thread. The reference runs it single-threaded."""
     import oracle
     m = k = A.shape[0]
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    cols = threads
-    while True:
+    threads = _cpu_threads()
+
+    def run_mt(cols):
         C = np.zeros((m, cols), np.float32, order="F")
         Bs = np.asfortranarray(B[:, :cols])
         t0 = time.perf_counter()
         oracle.sgemm_mt(threads, 0, 0, m, cols, k, 1.0, A, Bs, 0.0, C)
-        dt = time.perf_counter() - t0
-        if dt >= target_s or cols >= n:
-            break
-        cols = min(n, max(cols + threads, int(cols * min(8.0, 1.2 * target_s / max(dt, 1e-3)))))
-        cols = max(threads, (cols // threads) * threads)
-    flops = 2.0 * m * cols * k
-    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"SGEMM NN {m}x{cols}x{k} column slab of the {m}x{n}x{k} workload "
-                      f"({cols} of {n} output columns, {dt:.1f} s), naive cpu_gemm loop order with a double "
-                      f"accumulator, columns split over {threads} threads; rate extrapolates linearly in n"}
+        return time.perf_counter() - t0
+
+    use_ref = oracle.ref_available()
+
+    def run_1(cols):
+        C = np.zeros((m, cols), np.float32, order="F")
+        Bs = np.asfortranarray(B[:, :cols])
+        t0 = time.perf_counter()
+        if use_ref:
+            oracle.sgemm(0, 0, m, cols, k, 1.0, A, Bs, 0.0, C, reference=True)
+        else:
+            oracle.sgemm_mt(1, 0, 0, m, cols, k, 1.0, A, Bs, 0.0, C)
+        return time.perf_counter() - t0
+
+    v_mt, c_mt, t_mt = _slab_rate(run_mt, m, n, k, threads, target_s)
+    v_1, c_1, t_1 = _slab_rate(run_1, m, n, k, 1, max(2.0, target_s / 2))
+    model = _cpu_model()
+    return {"value": v_mt, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "host_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "sample": f"SGEMM NN {m}x{c_mt}x{k} column slab of the {m}x{n}x{k} workload "
+                      f"({c_mt} of {n} output columns, {t_mt:.1f} s), naive cpu_gemm loop order with a double "
+                      f"accumulator, columns split over {threads} threads on {model}; rate extrapolates linearly "
+                      f"in n",
+            "single_thread": {"value": v_1, "unit": "TFLOP/s", "cores": 1,
+                              "kind": "reference" if use_ref else "port",
+                              "sample": f"SGEMM NN {m}x{c_1}x{k} column slab ({c_1} of {n} columns, {t_1:.1f} s), "
+                                        + ("the reference's own cpu_gemm compiled from its headers"
+                                           if use_ref else "the restated cpu_gemm on one thread")}}
 
 
 def _time(stream, fn, steps: int, warmup: int) -> float:
@@ -584,6 +641,10 @@ def sgemm_line(a, h, ib, stream, dev):
         for _ in range(a.warmup):
             step()
         torch.cuda.synchronize()
+        # The library records HIP events around the main GEMM kernel of every call
+        # (the handle's stream, the one the kernel runs on; the split path's
+        # operand pack stays outside), over the same timed region as the call.
+        h.set_kernel_timing(True)
         start = torch.cuda.Event(enable_timing=True)
         stop = torch.cuda.Event(enable_timing=True)
         start.record(stream)
@@ -591,14 +652,20 @@ def sgemm_line(a, h, ib, stream, dev):
             step()
         stop.record(stream)
         torch.cuda.synchronize()
+        kern_total, kern_launches = h.kernel_timing()
+        h.set_kernel_timing(False)
         ms = start.elapsed_time(stop) / a.steps
+        kern_ms = kern_total / kern_launches if kern_launches > 0 else None
         impl = h.last_implementation()
         tflops = flops / (ms * 1e-3) / 1e12
         peak = PEAK_FP32_MFMA if path == "fp32" else PEAK_BF16_MFMA
-        executed = tflops if path == "fp32" else 3.0 * tflops
+        per_launch = flops if path == "fp32" else 3.0 * flops  # executed MFMA flops of one launch
+        executed = per_launch / (kern_ms * 1e-3) / 1e12 if kern_ms else None
         results[path] = {"implementation": impl, "ms": ms, "tflops": tflops,
                          "frac_of_fp32_mfma_peak": tflops / PEAK_FP32_MFMA,
-                         "executed_tflops": executed, "mfma_peak": peak, "mfma_frac": executed / peak}
+                         "kernel_ms": kern_ms, "kernel_launches": kern_launches,
+                         "executed_tflops": executed, "mfma_peak": peak,
+                         "mfma_frac": executed / peak if executed else None}
         if C64 is not None:
             err = (tC.double() - C64).abs()
             ratio = float((err / (k * 2.0 ** -24 * absAB)).max())
@@ -614,10 +681,20 @@ def sgemm_line(a, h, ib, stream, dev):
 
     head = "bf16x3" if "bf16x3" in results else "fp32"
     hr = results[head]
-    traffic = None
+    # HBM bytes per launch come from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # recorded in profiles/hbm_traffic.json (kernel name, workload, command); a
+    # live run has no counters, so the value is the recorded one for this kernel.
+    traffic_rec = {}
     tf = ROOT / "profiles" / "hbm_traffic.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get(hr["implementation"])
+        traffic_rec = json.loads(tf.read_text())
+
+    def traffic_of(impl):
+        r = traffic_rec.get(impl)
+        if isinstance(r, dict) and r.get("workload") == f"SGEMM {a.trans} {n}^3":
+            return r.get("bytes_per_launch")
+        return None
+    traffic = traffic_of(hr["implementation"])
     line = {
         "metric": "SGEMM TFLOP/s @8192: split-BF16 vs FP32-MFMA path, %peak, max err vs FP64",
         "value": hr["tflops"],
@@ -636,22 +713,75 @@ def sgemm_line(a, h, ib, stream, dev):
         "paths": results,
         "roofline": {"bound": "mfma", "achieved": hr["executed_tflops"], "peak": hr["mfma_peak"],
                      "unit": "TFLOP/s", "frac": hr["mfma_frac"], "traffic": traffic,
-                     "kernel": hr["implementation"],
-                     "note": "achieved = executed MFMA flops per launch (3 x 2mnk for split-BF16) / mean launch time"},
+                     "kernel": hr["implementation"], "kernel_ms": hr["kernel_ms"],
+                     "note": "achieved = executed MFMA flops per launch (2mnk; 3 x 2mnk for split-BF16) / mean "
+                             "duration of the main GEMM kernel (HIP events around it, same timed region); traffic "
+                             "= HBM bytes per launch from profiles/hbm_traffic.json"},
     }
     if "fp32" in results:
         r = results["fp32"]
-        t32 = None
-        if tf.exists():
-            t32 = json.loads(tf.read_text()).get(r["implementation"])
-        line["roofline_fp32"] = {"bound": "mfma", "achieved": r["tflops"], "peak": PEAK_FP32_MFMA,
-                                 "unit": "TFLOP/s", "frac": r["tflops"] / PEAK_FP32_MFMA, "traffic": t32,
-                                 "kernel": r["implementation"]}
+        line["roofline_fp32"] = {"bound": "mfma", "achieved": r["executed_tflops"], "peak": PEAK_FP32_MFMA,
+                                 "unit": "TFLOP/s", "frac": r["mfma_frac"],
+                                 "traffic": traffic_of(r["implementation"]),
+                                 "kernel": r["implementation"], "kernel_ms": r["kernel_ms"]}
         if "bf16x3" in results:
             line["split_over_fp32"] = results["bf16x3"]["tflops"] / r["tflops"]
     del C64, absAB, tA, tB, tC
     torch.cuda.empty_cache()
     return line, A, B
+
+
+# Pass criteria of the rows: the elementwise bound of the north-star contract for
+# GEMM products (max |err| / (k 2^-24 (|A||B|)) <= 4), FP32 accuracy (2^-23) for the
+# refined solve, and a normwise tolerance for the solves, level-3 and GEMV rows.
+ROW_TOL = {"strsm_plain": 1e-5, "strsm_refined": 2.0 ** -23, "ctrsm": 1e-5, "level3": 1e-5, "gemv": 1e-5,
+           "chain": 1e-4}
+
+
+def row_checks(line: dict):
+    """One (ok, text) summary per measured configuration and path."""
+    out = []
+
+    def fmt(name, r, err_key, tol):
+        err = r.get(err_key)
+        ok = err is None or (err == err and err <= tol)  # NaN fails
+        rate = f"{r['tflops']:.1f} TFLOP/s" if "tflops" in r else (f"{r['GBps']:.0f} GB/s" if "GBps" in r else "")
+        ms = f"{r['ms']:.3f} ms" if r.get("ms") is not None else ""
+        e = f"{err_key}={err:.3g} (<= {tol:.3g})" if err is not None else "unchecked"
+        out.append((ok, f"{'ok  ' if ok else 'FAIL'} {name}: {ms} {rate} {e} [{r.get('implementation', '')}]"))
+
+    for path, r in line.get("paths", {}).items():
+        fmt(f"sgemm_8192_{line['config'].get('workload', '').split()[1]}/{path}", r, "max_err_over_bound_unit", 4.0)
+    other = line.get("other_rows", {})
+    for name, rows in other.items():
+        if name.startswith("sgemm_"):
+            for path, r in rows.items():
+                fmt(f"{name}/{path}", r, "max_err_over_bound_unit", 4.0)
+        elif name == "cgemm_4096":
+            for op, paths in rows.items():
+                for path, r in paths.items():
+                    fmt(f"cgemm_4096_{op}/{path}", r, "max_err_over_bound_unit", 4.0)
+        elif name == "strsm_4096x1024":
+            for label, r in rows.items():
+                fmt(f"strsm_4096x1024/{label}", r, "forward_rel_err_fro",
+                    ROW_TOL["strsm_plain" if label == "plain" else "strsm_refined"])
+        elif name == "ctrsm_4096x1024":
+            for label, r in rows.items():
+                fmt(f"ctrsm_4096x1024/{label}", r, "forward_rel_err_fro", ROW_TOL["ctrsm"])
+        elif name == "chain_2048x512":
+            fmt("chain_2048x512/loss", rows, "loss_rel_err", ROW_TOL["chain"])
+            fmt("chain_2048x512/dH", rows, "dH_rel_err", ROW_TOL["chain"])
+        elif name == "level3":
+            for label, paths in rows.items():
+                for path, r in paths.items():
+                    fmt(f"{label}/{path}", r, "normwise_rel_err", ROW_TOL["level3"])
+        elif name == "gemv":
+            for label, r in rows.items():
+                fmt(label, r, "normwise_rel_err", ROW_TOL["gemv"])
+        elif name == "host_pointer":
+            for label, r in rows.items():
+                fmt(f"host/{label}", r, "none", 0.0)
+    return out
 
 
 def main():
@@ -723,7 +853,22 @@ def main():
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(A, B, a.size, a.cpu_seconds)
     h.close()
+    checks = row_checks(line)
+    failed = [t for ok, t in checks if not ok]
+    line["checks_failed"] = len(failed)
     print(json.dumps(line), flush=True)
+    # compact per-configuration summary (stderr, after the JSON line)
+    for _, text in checks:
+        print("[bench] " + text, file=sys.stderr)
+    cb = line.get("cpu_baseline")
+    if cb:
+        print(f"[bench] cpu_baseline: {cb['value'] * 1e3:.2f} GFLOP/s on {cb['cores']} threads, single thread "
+              f"{cb['single_thread']['value'] * 1e3:.3f} GFLOP/s ({cb['single_thread']['kind']}), {cb['cpu_model']}",
+              file=sys.stderr)
+    sys.stderr.flush()
+    if failed:
+        print(f"[bench] {len(failed)} configuration(s) outside their error bound", file=sys.stderr, flush=True)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -8,9 +8,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <vector>
 
 namespace icl {
+
+thread_local KernelTimer* t_active_timer = nullptr;
 
 // ---------------------------------------------------------------------------
 // Registry
@@ -140,7 +144,10 @@ static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, co
         const size_t bytes = v->workspace(&g);
         if (bytes) ws = h->scratch[iclblasContext::kSlotGemmWorkspace].get(bytes, h->stream);
     }
+    // A variant with a prepass marks its main kernel itself (icl_timer_mark).
+    if (!v->workspace) hip_check(icl_timer_mark(h->stream), "icl_timer_mark");
     hip_check(v->launch(&g, opa, opb, edge ? 1 : 0, ws, h->stream), v->name);
+    if (!v->workspace) hip_check(icl_timer_mark(h->stream), "icl_timer_mark");
     if (record) h->last_impl = v->name;
 }
 
@@ -589,6 +596,58 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
     h->last_impl = "ctrsm_panel256_inv";
 }
 
+// The panel solves of STRSM (panels of 256 unknowns in the canonical order):
+//  - kPanelSubst (strsm_panel256): each panel solved by the in-workgroup panel
+//    kernel, the later unknowns updated by one FP32 GEMM per panel;
+//  - kPanelInv (strsm_panel256_inv): the inverses W_p of the full panels formed
+//    up front, then per panel X_p = W_p B_p and the GEMM update;
+//  - kPanelFused (strsm_panel256_fused, the default): W_p and M_p = L_rp W_p
+//    (the later rows' coupling to the panel, times its inverse) formed up front,
+//    then per panel one grouped depth-256 launch [X_p; B_r] = [W_p; -M_p] B_p
+//    + [0; B_r].
+enum PanelMode { kPanelSubst, kPanelInv, kPanelFused };
+
+// Sum over the full panels p with later unknowns of 256 * rlen_p (the floats of
+// the M_p stack), rlen_p = order - 256 (p + 1).
+static size_t coupling_floats(int order)
+{
+    size_t sum = 0;
+    for (int pe = kPanel; pe <= order; pe += kPanel) sum += (size_t)kPanel * (size_t)(order - pe);
+    return sum;
+}
+
+// Workspace of a panel solve in floats: inverted 64 x 64 diagonal blocks, the
+// solution X (m x n), the panel inverses W_p of the full panels, then one region
+// holding first the identity stack W_p is formed from and afterwards (fused
+// mode) the M_p stack. Zero when the solve needs none (at most 64 unknowns).
+static size_t panel_ws_floats(int order, int m, int n, PanelMode mode)
+{
+    if (order <= kLeaf) return 0;
+    const size_t nblk = (size_t)(order + kLeaf - 1) / kLeaf;
+    const size_t nfull = mode == kPanelSubst ? 0 : (size_t)order / kPanel;
+    const size_t ident = nfull ? (size_t)order * kPanel : 0;
+    const size_t shared = mode == kPanelFused ? std::max(ident, coupling_floats(order)) : ident;
+    return nblk * kLeaf * kLeaf + (size_t)m * n + nfull * kPanel * kPanel + shared;
+}
+
+// Queues the problems of `g` (any count) as launches of at most
+// ICL_PANEL_GEMM_MAX problems each.
+static void panel_gemm_all(iclblasContext* h, std::vector<icl_panel_gemm_problem>& probs)
+{
+    for (size_t i = 0; i < probs.size(); i += ICL_PANEL_GEMM_MAX) {
+        icl_panel_gemm_args g{};
+        g.nprob = (int)std::min<size_t>(ICL_PANEL_GEMM_MAX, probs.size() - i);
+        for (int j = 0; j < g.nprob; ++j) g.prob[j] = probs[i + j];
+        hip_check(icl_panel_gemm(&g, h->stream), "panel_gemm");
+    }
+}
+
+// The TRSM workspace: the handle's grow-only buffer.
+static void* trsm_workspace(iclblasContext* h, size_t bytes)
+{
+    return h->scratch[iclblasContext::kSlotTrsmWorkspace].get(bytes, h->stream);
+}
+
 void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, float alpha,
                const float* A, long lda, float* B, long ldb, bool record, bool reuse_inverses)
 {
@@ -653,7 +712,9 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
         const std::string& f = h->forced_impl;
         const bool dataflow = f == "strsm_dataflow64";
         const bool panel = !dataflow && f != "strsm_blocked_inv64"; // the panel solves (the default among them:)
-        const bool panel_inv = panel && f != "strsm_panel256";       // with inverted panels
+        const PanelMode mode = f == "strsm_panel256" ? kPanelSubst : f == "strsm_panel256_inv" ? kPanelInv
+                                                                                              : kPanelFused;
+        const bool panel_inv = mode != kPanelSubst;
         if (panel) {
             // X in original order with B's shape (m x n, ld m); the canonical X view
             // mirrors B's. Panels of 256 unknowns are solved one after the other,
@@ -663,14 +724,16 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
             // inverse of every full 256 x 256 diagonal panel at once (the panel
             // kernel on identity right-hand sides, all panels in one launch: they
             // do not depend on B) and then solves each panel with one GEMM
-            // X_p = W_p B_p, which spreads over the whole GPU.
+            // X_p = W_p B_p, which spreads over the whole GPU; the fused mode
+            // (PanelMode) also forms the M_p up front and runs each panel as one
+            // grouped launch.
             const int nfull = panel_inv ? order / kPanel : 0;
-            const size_t w_elems = (size_t)nfull * kPanel * kPanel, e_elems = nfull ? (size_t)order * kPanel : 0;
-            float* ws = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
-                (inv_elems + (size_t)m * n + w_elems + e_elems) * sizeof(float), h->stream));
+            const size_t w_elems = (size_t)nfull * kPanel * kPanel;
+            float* ws = static_cast<float*>(trsm_workspace(h, panel_ws_floats(order, m, n, mode) * sizeof(float)));
             float* Xb = ws + inv_elems;
             float* W = Xb + (size_t)m * n;
-            float* E = W + w_elems;
+            float* E = W + w_elems; // identity stack, then (fused) the M_p stack
+            float* Mst = E;
             a.invD = ws;
             if (t.left) {
                 a.X = t.fwd ? Xb : Xb + e;
@@ -701,6 +764,76 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
                 ai.xv_sj = t.fwd ? col_s : -col_s;
                 ai.x_batch = (long)kPanel * kPanel;
                 hip_check(icl_trsm_panel_batch(&ai, nfull, h->stream), "trsm_panel_batch");
+            }
+            // op(A)(i, j) = A[i * si + j * sk] in original coordinates
+            auto opa = [&](int i0, int j0) { return A + (long)i0 * t.si + (long)j0 * t.sk; };
+            if (mode == kPanelFused) {
+                // Panel p (full, with rlen_p later unknowns) in original coordinates:
+                // left  X_p = W_p B_p, B_r -= M_p B_p with M_p = op(A)[r, p] W_p
+                //       (rlen_p x 256, ld rlen_p);
+                // right X_p = B_p W_p, B_r -= B_p N_p with N_p = W_p op(A)[p, r]
+                //       (256 x rlen_p, ld 256).
+                // As panel_gemm problems D(i, l) = sum_k P(i, k) Q(k, l), i the
+                // unknowns and l the lines (B's columns for left, rows for right).
+                std::vector<icl_panel_gemm_problem> probs;
+                size_t moff = 0;
+                for (int p = 0; p < nfull; ++p) {
+                    const int ps = p * kPanel, pe = ps + kPanel, rlen = order - pe;
+                    if (rlen == 0) break;
+                    const int p_lo = t.fwd ? ps : order - pe, r_lo = t.fwd ? pe : 0;
+                    const float* Wp = W + (size_t)p * kPanel * kPanel;
+                    icl_panel_gemm_problem q{};
+                    q.alpha = 1.0f;
+                    if (t.left) { // M(i, c) = sum_k op(A)(r_lo + i, p_lo + k) W(k, c)
+                        q.rows = rlen, q.lines = kPanel;
+                        q.P = opa(r_lo, p_lo), q.p_is = t.si, q.p_ks = t.sk;
+                        q.Q = Wp, q.q_ks = 1, q.q_ls = kPanel;
+                        q.D = Mst + moff, q.d_is = 1, q.d_ls = rlen;
+                    } else { // N(j, i) = sum_k W(j, k) op(A)(p_lo + k, r_lo + i), W(j, k) = Wp[j + 256 k]
+                        q.rows = kPanel, q.lines = rlen;
+                        q.P = Wp, q.p_is = 1, q.p_ks = kPanel;
+                        q.Q = opa(p_lo, r_lo), q.q_ks = t.si, q.q_ls = t.sk;
+                        q.D = Mst + moff, q.d_is = 1, q.d_ls = kPanel;
+                    }
+                    if (!reuse_inverses) probs.push_back(q);
+                    moff += (size_t)kPanel * rlen;
+                }
+                panel_gemm_all(h, probs);
+                moff = 0;
+                for (int ps = 0; ps < order; ps += kPanel) {
+                    const int pe = std::min(order, ps + kPanel);
+                    const int plen = pe - ps, rlen = order - pe;
+                    const int p_lo = t.fwd ? ps : order - pe, r_lo = t.fwd ? pe : 0;
+                    if (plen < kPanel) { // the last, partial panel: no later unknowns
+                        hip_check(icl_trsm_panel(&a, ps, pe, h->stream), "trsm_panel");
+                        break;
+                    }
+                    const float* Wp = W + (size_t)(ps / kPanel) * kPanel * kPanel;
+                    icl_panel_gemm_args g{};
+                    g.nprob = rlen > 0 ? 2 : 1;
+                    icl_panel_gemm_problem& x = g.prob[0];
+                    icl_panel_gemm_problem& u = g.prob[1];
+                    x.rows = kPanel, x.alpha = 1.0f, x.beta = 0.0f, x.P = Wp;
+                    u.rows = rlen, u.alpha = -1.0f, u.beta = 1.0f, u.P = Mst + moff;
+                    if (t.left) {
+                        x.lines = u.lines = n;
+                        x.Q = u.Q = B + p_lo, x.q_ks = u.q_ks = 1, x.q_ls = u.q_ls = ldb;
+                        x.p_is = u.p_is = 1, x.p_ks = kPanel, u.p_ks = rlen;
+                        x.D = Xb + p_lo, x.d_is = 1, x.d_ls = m;
+                        u.D = B + r_lo, u.d_is = 1, u.d_ls = ldb;
+                    } else {
+                        x.lines = u.lines = m;
+                        x.Q = u.Q = B + (long)p_lo * ldb, x.q_ks = u.q_ks = ldb, x.q_ls = u.q_ls = 1;
+                        x.p_is = u.p_is = kPanel, x.p_ks = u.p_ks = 1;
+                        x.D = Xb + (long)p_lo * m, x.d_is = m, x.d_ls = 1;
+                        u.D = B + (long)r_lo * ldb, u.d_is = ldb, u.d_ls = 1;
+                    }
+                    hip_check(icl_panel_gemm(&g, h->stream), "panel_gemm");
+                    moff += (size_t)kPanel * rlen;
+                }
+                hip_check(icl_axpby_2d(m, n, alpha, Xb, m, 0.0f, B, ldb, h->stream), "axpby");
+                if (record) h->last_impl = "strsm_panel256_fused";
+                return;
             }
             for (int ps = 0; ps < order; ps += kPanel) {
                 const int pe = std::min(order, ps + kPanel);
@@ -743,9 +876,16 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
                 void* w = nullptr;
                 hip_check(hipMalloc(&w, 256), "hipMalloc");
                 h->trsm_error = static_cast<int*>(w);
-                hip_check(icl_axpby_2d(64, 1, 0.0f, nullptr, 0, 0.0f, static_cast<float*>(w), 64, h->stream), "axpby");
             }
+            // The word reports this solve only: cleared on the stream before it.
+            hip_check(icl_axpby_2d(64, 1, 0.0f, nullptr, 0, 0.0f, reinterpret_cast<float*>(h->trsm_error), 64,
+                                   h->stream),
+                      "axpby");
+            h->trsm_error_pending = true;
             a.error_word = h->trsm_error;
+            // Diagnostics: ICLBLAS_DATAFLOW_WAIT_POLLS overrides the polls a wait
+            // makes before it aborts (a negative value aborts the first wait).
+            if (const char* env = std::getenv("ICLBLAS_DATAFLOW_WAIT_POLLS")) a.wait_polls = std::atoi(env);
             hip_check(icl_trsm_invert_blocks(&a, h->stream), "trsm_invert_blocks");
             hip_check(icl_trsm_dataflow(&a, alpha, h->stream), "trsm_dataflow");
             if (record) h->last_impl = "strsm_dataflow64";
@@ -766,7 +906,8 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
 
 void trsm_check_error(iclblasContext* h)
 {
-    if (!h->trsm_error) return;
+    if (!h->trsm_error || !h->trsm_error_pending) return;
+    h->trsm_error_pending = false;
     int e = 0;
     hip_check(hipMemcpy(&e, h->trsm_error, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
     if (e == 0) return;
@@ -908,3 +1049,8 @@ void strsm_backward_dev(iclblasContext* h, int side, int uplo, int trans, int di
 }
 
 } // namespace icl
+
+extern "C" hipError_t icl_timer_mark(hipStream_t stream)
+{
+    return icl::t_active_timer ? icl::t_active_timer->mark(stream) : hipSuccess;
+}

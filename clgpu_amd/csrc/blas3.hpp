@@ -91,9 +91,10 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
 void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m, int n, const float alpha[2],
                const float* A, long lda, float* B, long ldb);
 
-// Host memory mode only (after the stream is synchronised): throws
-// EXECUTION_FAILED, and clears the word, when a dataflow STRSM of this handle
-// aborted a wait.
+// After the handle's stream is synchronised: when a dataflow STRSM was queued
+// since the last check, reads its error word and throws EXECUTION_FAILED (and
+// clears the word) when that solve aborted a wait. Host-pointer calls check at
+// their end; in the device-pointer mode iclblasSynchronize reports it.
 void trsm_check_error(iclblasContext* h);
 
 // STRSM + FP64-residual iterative refinement. ws0/ws1: m*n floats each.

@@ -35,8 +35,9 @@ namespace {
 // solves plus GEMM updates. CTRSM above 64 unknowns: the panel solve with
 // inverted 256 x 256 panels (default) and the recursive split whose leaves are
 // products with the inverted 64 x 64 diagonal blocks.
-const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_dataflow64", "strsm_panel256", "strsm_panel256_inv",
-                                  "strsm_recursive", "ctrsm_panel256_inv", "ctrsm_recursive_inv64"};
+const char* const kTrsmImpls[] = {"strsm_blocked_inv64", "strsm_dataflow64",      "strsm_panel256",
+                                  "strsm_panel256_inv",  "strsm_panel256_fused",  "strsm_recursive",
+                                  "ctrsm_panel256_inv",  "ctrsm_recursive_inv64"};
 
 // Shared GEMM validation; *work is set when the call has work to do.
 iclblasStatus_t gemm_checks(iclblasHandle_t handle, int transa, int transb, int m, int n, int k, const void* alpha,
@@ -52,25 +53,6 @@ iclblasStatus_t gemm_checks(iclblasHandle_t handle, int transa, int transb, int 
     const int rows_a = transa == ICLBLAS_OP_N ? m : k;
     const int rows_b = transb == ICLBLAS_OP_N ? k : n;
     if (lda < (rows_a > 1 ? rows_a : 1) || ldb < (rows_b > 1 ? rows_b : 1) || ldc < m) return ICLBLAS_STATUS_INVALID_VALUE;
-    if (!iclblasContext::valid(handle)) return ICLBLAS_STATUS_INVALID_VALUE;
-    *work = true;
-    return ICLBLAS_STATUS_SUCCESS;
-}
-
-iclblasStatus_t trsm_checks(iclblasHandle_t handle, int side, int uplo, int trans, int diag, int m, int n,
-                            const float* alpha, const float* A, int lda, const float* B, int ldb, bool* work)
-{
-    *work = false;
-    if (m == 0 || n == 0) return ICLBLAS_STATUS_SUCCESS;
-    if (!alpha) return ICLBLAS_STATUS_INVALID_VALUE;
-    if (n < 0 || m < 0) return ICLBLAS_STATUS_INVALID_VALUE;
-    if (!A || lda <= 0 || !B || ldb <= 0) return ICLBLAS_STATUS_INVALID_VALUE;
-    if ((side != ICLBLAS_SIDE_LEFT && side != ICLBLAS_SIDE_RIGHT) ||
-        (uplo != ICLBLAS_FILL_MODE_UPPER && uplo != ICLBLAS_FILL_MODE_LOWER) || !op_ok(trans) ||
-        (diag != ICLBLAS_DIAG_NON_UNIT && diag != ICLBLAS_DIAG_UNIT))
-        return ICLBLAS_STATUS_INVALID_VALUE;
-    const int order = side == ICLBLAS_SIDE_LEFT ? m : n;
-    if (lda < order || ldb < m) return ICLBLAS_STATUS_INVALID_VALUE;
     if (!iclblasContext::valid(handle)) return ICLBLAS_STATUS_INVALID_VALUE;
     *work = true;
     return ICLBLAS_STATUS_SUCCESS;
@@ -339,6 +321,7 @@ ICLBLAS_API iclblasStatus_t iclblasSynchronize(iclblasHandle_t handle)
     return guarded([&] {
         HandleLock lock(handle);
         icl::hip_check(hipStreamSynchronize(handle->stream), "hipStreamSynchronize");
+        icl::trsm_check_error(handle); // an aborted dataflow STRSM queued in the device-pointer mode
     });
 }
 
@@ -395,6 +378,39 @@ ICLBLAS_API iclblasStatus_t iclblasGetLastRefinementSweeps(iclblasHandle_t handl
     std::lock_guard<std::mutex> g(handle->mu);
     *sweeps = handle->last_sweeps;
     return ICLBLAS_STATUS_SUCCESS;
+}
+
+ICLBLAS_API iclblasStatus_t iclblasSetKernelTiming(iclblasHandle_t handle, int enable)
+{
+    if (!iclblasContext::valid(handle)) return ICLBLAS_STATUS_INVALID_VALUE;
+    return guarded([&] {
+        HandleLock lock(handle);
+        icl::hip_check(hipStreamSynchronize(handle->stream), "hipStreamSynchronize");
+        handle->timer.on = enable != 0;
+        handle->timer.used = 0;
+        handle->timer.dropped = 0;
+    });
+}
+
+ICLBLAS_API iclblasStatus_t iclblasGetKernelTiming(iclblasHandle_t handle, float* total_ms, int* launches)
+{
+    if (!iclblasContext::valid(handle) || !total_ms || !launches) return ICLBLAS_STATUS_INVALID_VALUE;
+    return guarded([&] {
+        HandleLock lock(handle);
+        icl::KernelTimer& t = handle->timer;
+        icl::hip_check(hipStreamSynchronize(handle->stream), "hipStreamSynchronize");
+        double sum = 0.0;
+        for (size_t i = 0; i + 1 < t.used; i += 2) {
+            icl::hip_check(hipEventSynchronize(t.ev[i + 1]), "hipEventSynchronize");
+            float ms = 0.0f;
+            icl::hip_check(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]), "hipEventElapsedTime");
+            sum += ms;
+        }
+        *total_ms = (float)sum;
+        *launches = (int)(t.used / 2); // launches beyond the event cap are not in the sum
+        t.used = 0;
+        t.dropped = 0;
+    });
 }
 
 // ---------------------------------------------------------------------------

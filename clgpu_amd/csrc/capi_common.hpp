@@ -34,6 +34,28 @@ iclblasStatus_t guarded(F&& f)
 
 inline bool op_ok(int op) { return op == ICLBLAS_OP_N || op == ICLBLAS_OP_T || op == ICLBLAS_OP_C; }
 
+// STRSM / CTRSM validation in the reference's order (iclBLAS_S.cpp:500-510:
+// m == 0 or n == 0 -> SUCCESS first; then iclBLASImpl.hpp:108-233), followed by
+// the checks the reference does not make; *work is set when the call has work.
+inline iclblasStatus_t trsm_checks(iclblasHandle_t handle, int side, int uplo, int trans, int diag, int m, int n,
+                                   const void* alpha, const void* A, int lda, const void* B, int ldb, bool* work)
+{
+    *work = false;
+    if (m == 0 || n == 0) return ICLBLAS_STATUS_SUCCESS;
+    if (!alpha) return ICLBLAS_STATUS_INVALID_VALUE;
+    if (n < 0 || m < 0) return ICLBLAS_STATUS_INVALID_VALUE;
+    if (!A || lda <= 0 || !B || ldb <= 0) return ICLBLAS_STATUS_INVALID_VALUE;
+    if ((side != ICLBLAS_SIDE_LEFT && side != ICLBLAS_SIDE_RIGHT) ||
+        (uplo != ICLBLAS_FILL_MODE_UPPER && uplo != ICLBLAS_FILL_MODE_LOWER) || !op_ok(trans) ||
+        (diag != ICLBLAS_DIAG_NON_UNIT && diag != ICLBLAS_DIAG_UNIT))
+        return ICLBLAS_STATUS_INVALID_VALUE;
+    const int order = side == ICLBLAS_SIDE_LEFT ? m : n;
+    if (lda < order || ldb < m) return ICLBLAS_STATUS_INVALID_VALUE;
+    if (!iclblasContext::valid(handle)) return ICLBLAS_STATUS_INVALID_VALUE;
+    *work = true;
+    return ICLBLAS_STATUS_SUCCESS;
+}
+
 // Number of elements a column-major rows x cols matrix with leading dimension ld spans.
 inline size_t extent(int rows, int cols, int ld) { return rows == 0 || cols == 0 ? 0 : (size_t)(cols - 1) * ld + rows; }
 
@@ -46,8 +68,13 @@ struct HandleLock {
             h->mu.unlock();
             throw status_error(ICLBLAS_STATUS_EXECUTION_FAILED, "hipSetDevice");
         }
+        icl::t_active_timer = h->timer.on ? &h->timer : nullptr;
     }
-    ~HandleLock() { h->mu.unlock(); }
+    ~HandleLock()
+    {
+        icl::t_active_timer = nullptr;
+        h->mu.unlock();
+    }
 };
 
 // Host-memory staging for the synchronous host-pointer mode, through the

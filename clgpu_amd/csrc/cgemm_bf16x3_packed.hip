@@ -255,14 +255,18 @@ hipError_t cpacked_launch(const icl_gemm_args* g, int opa, int opb, int edge, vo
     if (e != hipSuccess) return e;
     using C = CDmaCfg<BM, BN, WVM, WVN>;
     const bool even = g->M % BM == 0 && g->N % BN == 0;
+    if ((e = icl_timer_mark(s)) != hipSuccess) return e;
     if (uses_level3(g))
-        return even ? launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false, true>,
-                                                              g, s, p)
-                    : launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true, true>,
-                                                              g, s, p);
-    if (even)
-        return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, g, s, p);
-    return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true>, g, s, p);
+        e = even ? launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false, true>, g,
+                                                           s, p)
+                 : launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true, true>, g,
+                                                           s, p);
+    else if (even)
+        e = launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, g, s, p);
+    else
+        e = launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(cgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true>, g, s, p);
+    if (e != hipSuccess) return e;
+    return icl_timer_mark(s);
 }
 
 } // namespace

@@ -12,6 +12,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "iclBLAS.h"
 #include "iclBLAS_ext.h"
@@ -134,6 +135,49 @@ struct CopyLane {
     }
 };
 
+// Opt-in timing of the main kernel of each GEMM launch (iclblasSetKernelTiming):
+// a begin/end HIP event pair recorded on the launch stream around that kernel
+// alone. Events are created on first use and reused; at most kMaxEvents are
+// recorded between two reads (later marks are counted as dropped).
+struct KernelTimer {
+    static constexpr size_t kMaxEvents = 8192;
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    long dropped = 0;
+    KernelTimer() = default;
+    KernelTimer(const KernelTimer&) = delete;
+    KernelTimer& operator=(const KernelTimer&) = delete;
+    ~KernelTimer() { release(); }
+
+    hipError_t mark(hipStream_t s)
+    {
+        if (used == kMaxEvents) {
+            ++dropped;
+            return hipSuccess;
+        }
+        if (used == ev.size()) {
+            hipEvent_t e = nullptr;
+            const hipError_t err = hipEventCreate(&e);
+            if (err != hipSuccess) return err;
+            ev.push_back(e);
+        }
+        return hipEventRecord(ev[used++], s);
+    }
+
+    void release()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        used = 0;
+        dropped = 0;
+    }
+};
+
+// The timer of the handle whose call the current thread is running (set for the
+// duration of a call when that handle's timing is on), read by icl_timer_mark.
+extern thread_local KernelTimer* t_active_timer;
+
 } // namespace icl
 
 struct iclblasContext {
@@ -160,6 +204,8 @@ struct iclblasContext {
     icl::PinnedStager stager;
     icl::CopyLane copy;
     int* trsm_error = nullptr; // device word set by an aborted dataflow STRSM (allocated on first use)
+    bool trsm_error_pending = false; // a dataflow STRSM was queued since the word was last read
+    icl::KernelTimer timer;
 
     static bool valid(const iclblasContext* h) { return h != nullptr && h->tag == kTag; }
 };

@@ -79,6 +79,13 @@ typedef struct icl_gemm_variant {
     int level3;        /* implements the level-3 fields write_tri, kband, cflags */
 } icl_gemm_variant;
 
+/* Kernel timing hook. When the calling thread's handle has kernel timing on,
+ * records a HIP event on `stream`; otherwise does nothing. The host layer marks
+ * the launch of a variant without a workspace; a variant with a prepass marks
+ * its main kernel itself (right before and right after that launch), so the
+ * prepass stays outside the timed pair. */
+hipError_t icl_timer_mark(hipStream_t stream);
+
 /* Registered GEMM implementations. Kernel translation units register their
  * tables at library load (icl_register_gemm_variants); the host layer reads
  * them back with icl_gemm_variant_count / icl_gemm_variant_at, ordered by name. */
@@ -150,6 +157,33 @@ typedef struct icl_panel_apply_args {
 } icl_panel_apply_args;
 hipError_t icl_panel_apply(const icl_panel_apply_args* a, hipStream_t stream);
 
+/* Grouped products of depth 256 for the panel solves of the blocked TRSMs: up to
+ * ICL_PANEL_GEMM_MAX problems in one launch, each
+ *   D(i, l) = alpha * sum_{k < 256} P(i, k) Q(k, l) + beta * D(i, l)
+ * for i < rows, l < lines (D is not read when beta == 0), with
+ * P(i, k) = P[i * p_is + k * p_ks], Q(k, l) = Q[k * q_ks + l * q_ls] and
+ * D(i, l) = D[i * d_is + l * d_ls] in elements. Every problem of a launch must
+ * have the same contiguous dimensions: p_is == 1 or p_ks == 1, q_ks == 1 or
+ * q_ls == 1, d_is == 1 or d_ls == 1, alike in all problems. No D may overlap a
+ * P or Q of the launch. tile0 is set by icl_panel_gemm. */
+#define ICL_PANEL_GEMM_MAX 16
+typedef struct icl_panel_gemm_problem {
+    int rows, lines;
+    const float* P;
+    long p_is, p_ks;
+    const float* Q;
+    long q_ks, q_ls;
+    float* D;
+    long d_is, d_ls;
+    float alpha, beta;
+    int tile0;
+} icl_panel_gemm_problem;
+typedef struct icl_panel_gemm_args {
+    int nprob;
+    icl_panel_gemm_problem prob[ICL_PANEL_GEMM_MAX];
+} icl_panel_gemm_args;
+hipError_t icl_panel_gemm(icl_panel_gemm_args* a, hipStream_t stream);
+
 /* Inverses of the 256 x 256 diagonal panels of the canonical complex system
  * (panel p = unknowns [256 p, 256 p + plen), plen = min(256, nsys - 256 p)),
  * one workgroup per 64-wide column block of a panel, from the inverted 64 x 64
@@ -205,6 +239,8 @@ typedef struct icl_trsm_blocked_args {
     int* sync;       /* dataflow solve only: icl_trsm_dataflow_sync_ints(...) ints, zeroed by invert_blocks */
     int* error_word; /* dataflow solve only: set to 1 (never cleared by the kernels) when a wait is aborted */
     long x_batch;    /* icl_trsm_panel_batch only: X offset (elements) between consecutive panels */
+    int wait_polls;  /* dataflow solve only: polls of a flag per wait before the wait aborts; 0 = the
+                      * default bound (65536), < 0 = abort the first wait without polling */
 } icl_trsm_blocked_args;
 hipError_t icl_trsm_invert_blocks(const icl_trsm_blocked_args* a, hipStream_t stream);
 hipError_t icl_trsm_step(const icl_trsm_blocked_args* a, int block, hipStream_t stream);
@@ -218,7 +254,7 @@ hipError_t icl_trsm_scatter(const icl_trsm_blocked_args* a, float alpha, hipStre
  * then X_bc = inv(L_bb) (B_bc - sum), stores it, sets its flag and writes
  * alpha * X_bc through the view of B. invert_blocks (same stream, just before)
  * zeroes a->sync (per-tile flags, the ticket counter, then the abort word). A
- * wait that exceeds its spin bound sets the abort word (the last int), after
+ * wait that exceeds a->wait_polls polls sets the abort word (the last int), after
  * which every wait returns at once, so the launch always drains; the result is
  * then invalid, and *error_word is set for the host to report. */
 int icl_trsm_dataflow_sync_ints(int nsys, int nlines);

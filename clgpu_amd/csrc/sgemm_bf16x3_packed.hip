@@ -262,20 +262,24 @@ hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, voi
     const PackSrc pa{g->A, g->lda, g->M, g->K, p.Mpad, opa != 0, Ap};
     // op(B)(k, n) as rows n: N -> B[k + n*ldb] (k contiguous), T/C -> B[n + k*ldb]
     const PackSrc pb{g->B, g->ldb, g->N, g->K, p.Npad, opb == 0, Bp};
-    const hipError_t e = pack(pa, pb, p.slabs, s);
+    hipError_t e = pack(pa, pb, p.slabs, s);
     if (e != hipSuccess) return e;
     const bool even = g->M % BM == 0 && g->N % BN == 0;
     (void)edge;
     // launch_gemm sizes the grid from (M, N); the packed grid is the same.
     using C = DmaCfg<BM, BN, WVM, WVN>;
+    if ((e = icl_timer_mark(s)) != hipSuccess) return e;
     if (uses_level3(g))
-        return even ? launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false, true>,
-                                                              g, s, p)
-                    : launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true, true>,
-                                                              g, s, p);
-    if (even)
-        return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, g, s, p);
-    return launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true>, g, s, p);
+        e = even ? launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false, true>, g,
+                                                           s, p)
+                 : launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true, true>, g,
+                                                           s, p);
+    else if (even)
+        e = launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, g, s, p);
+    else
+        e = launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, true>, g, s, p);
+    if (e != hipSuccess) return e;
+    return icl_timer_mark(s);
 }
 
 } // namespace

@@ -571,8 +571,9 @@ __global__ void __launch_bounds__(256) trsm_step_kernel(icl_trsm_blocked_args a,
 // waits for tiles whose tickets are smaller; those were taken by workgroups
 // that are already running and themselves wait only on smaller tickets, so
 // every wait ends (by induction from ticket 0) however many workgroups are
-// resident. Each wait is also bounded: past kDfSpin polls it sets the abort
-// word, and every later wait returns at once, so the launch always drains.
+// resident. Each wait is also bounded: past a.wait_polls polls (kDfSpin by
+// default) it sets the abort word, and every later wait returns at once, so the
+// launch always drains.
 // A wave polls up to 64 flags at a time and skips the polls (and acquires) of
 // the steps whose producers it has already seen finished.
 // A producer publishes a tile from one wave, with a release fence at agent
@@ -583,7 +584,7 @@ __global__ void __launch_bounds__(256) trsm_step_kernel(icl_trsm_blocked_args a,
 // contiguous in B fastest (lines when LINES_CONTIG, rows otherwise).
 // ---------------------------------------------------------------------------
 constexpr int kDfLd = kNB + 1;                                      // pitch of the [k][r] LDS tiles
-constexpr int kDfSpin = 1 << 16;                                    // polls per wait before aborting
+constexpr int kDfSpin = 1 << 16;                                    // default polls per wait before aborting
 constexpr size_t kDfLds = sizeof(float) * ((size_t)2 * kNB * kDfLd + (size_t)kNB * kLdD);
 
 __device__ __forceinline__ int df_poll(const int* p)
@@ -596,12 +597,18 @@ __device__ __forceinline__ int df_poll(const int* p)
 // at once, one per lane, and returns the end of the run of set flags starting at
 // j, so the steps up to it need no further poll. One acquire at agent scope
 // covers the whole run. Returns n when the solve was aborted (then nothing may be
-// assumed about the tiles).
+// assumed about the tiles). At most max_polls polls: the bound is tested before
+// each poll, so max_polls == 0 aborts the wait at once.
 __device__ __forceinline__ int df_ready(const int* flags, long stride, int j, int n, int lane, int* abort_word,
-                                        int* error_word)
+                                        int* error_word, int max_polls)
 {
     const int jj = j + lane;
     for (int it = 0;; ++it) {
+        if (it >= max_polls) {
+            __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(error_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return n;
+        }
         const int f = jj < n ? df_poll(flags + (long)jj * stride) : 0;
         const unsigned long long set = __ballot(f != 0);
         const int run = ~set == 0ull ? 64 : __builtin_ctzll(~set);
@@ -610,11 +617,6 @@ __device__ __forceinline__ int df_ready(const int* flags, long stride, int j, in
             return j + run;
         }
         if (df_poll(abort_word) != 0) return n;
-        if (it >= kDfSpin) {
-            __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(error_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return n;
-        }
         __builtin_amdgcn_s_sleep(2);
     }
 }
@@ -640,6 +642,7 @@ __global__ void __launch_bounds__(256) trsm_dataflow_kernel(icl_trsm_blocked_arg
     int* const flags = a.sync;
     int* const ticket = a.sync + nblk * ncb;
     int* const abort_word = ticket + 1;
+    const int max_polls = a.wait_polls == 0 ? kDfSpin : (a.wait_polls < 0 ? 0 : a.wait_polls);
     if (tid == 0) s_ticket = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int t = s_ticket;
@@ -689,7 +692,7 @@ __global__ void __launch_bounds__(256) trsm_dataflow_kernel(icl_trsm_blocked_arg
     int ready = 0; // tiles (j, c) with j < ready are known to be published (uniform per wave)
     auto fetch = [&](int j) {
         lt.load(l_row + (long)j * kNB * a.l_sk, a.l_si, a.l_sk, a.nsys - r0, kNB, tid, l_diag);
-        if (j >= ready) ready = df_ready(flags + c, ncb, j, b, lane, abort_word, a.error_word);
+        if (j >= ready) ready = df_ready(flags + c, ncb, j, b, lane, abort_word, a.error_word, max_polls);
         xt.load(xtile(j), s_l, s_k, kNB, kNB, tid, a.X);
     };
     auto multiply = [&]() {

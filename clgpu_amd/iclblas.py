@@ -102,6 +102,8 @@ _SIGS = {
     "iclblasGetLastImplementation": [_P, ct.c_char_p, _I],
     "iclblasGetLastRefinementSweeps": [_P, ct.POINTER(_I)],
     "iclblasGetImplementations": [ct.c_char_p, _I],
+    "iclblasSetKernelTiming": [_P, _I],
+    "iclblasGetKernelTiming": [_P, _FP, ct.POINTER(_I)],
 }
 
 
@@ -331,6 +333,15 @@ class Handle:
         v = ct.c_int()
         check(load().iclblasGetLastRefinementSweeps(self.h, ct.byref(v)), "iclblasGetLastRefinementSweeps")
         return v.value
+
+    def set_kernel_timing(self, on: bool):
+        check(load().iclblasSetKernelTiming(self.h, 1 if on else 0), "iclblasSetKernelTiming")
+
+    def kernel_timing(self) -> tuple[float, int]:
+        """(summed ms, launches) of the main GEMM kernels timed since the last read."""
+        ms, n = ct.c_float(), ct.c_int()
+        check(load().iclblasGetKernelTiming(self.h, ct.byref(ms), ct.byref(n)), "iclblasGetKernelTiming")
+        return ms.value, n.value
 
 
 def exported_symbols() -> list[str]:

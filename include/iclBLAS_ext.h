@@ -49,7 +49,10 @@ ICLBLAS_API iclblasStatus_t iclblasGetMathMode(iclblasHandle_t handle, iclblasMa
  * (NULL = the null stream). The handle does not own a stream set this way. */
 ICLBLAS_API iclblasStatus_t iclblasSetStream(iclblasHandle_t handle, void* stream);
 ICLBLAS_API iclblasStatus_t iclblasGetStream(iclblasHandle_t handle, void** stream);
-/* Waits for all work queued on the handle's stream. */
+/* Waits for all work queued on the handle's stream. Returns
+ * ICLBLAS_STATUS_EXECUTION_FAILED when a solve queued since the last
+ * synchronising call failed on the device (the dataflow STRSM aborting a wait;
+ * host-pointer calls report that themselves). */
 ICLBLAS_API iclblasStatus_t iclblasSynchronize(iclblasHandle_t handle);
 
 /* Forces the named implementation for subsequent calls of the function it
@@ -104,6 +107,16 @@ ICLBLAS_API iclblasStatus_t iclblasStrsmBackward(iclblasHandle_t handle, iclblas
                                                  iclblasDiagType_t diag, int m, int n, const float* alpha,
                                                  float* A, int lda, float* X, int ldx, float* dX, int lddx,
                                                  float* dA, int ldda, float* dB, int lddb);
+
+/* Kernel timing (a measurement aid). While enabled, every GEMM the handle runs
+ * records a pair of HIP events on the handle's stream around its main kernel
+ * alone (the operand pack of the split-BF16 kernels runs outside the pair).
+ * iclblasSetKernelTiming waits for the handle's stream and starts a new sum.
+ * iclblasGetKernelTiming waits for the stream, returns the summed duration of
+ * the recorded pairs and their number, and starts a new sum; at most 4096
+ * launches are timed between two reads. */
+ICLBLAS_API iclblasStatus_t iclblasSetKernelTiming(iclblasHandle_t handle, int enable);
+ICLBLAS_API iclblasStatus_t iclblasGetKernelTiming(iclblasHandle_t handle, float* total_ms, int* launches);
 
 #ifdef __cplusplus
 } /* extern "C" */

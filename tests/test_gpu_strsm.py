@@ -54,6 +54,7 @@ def _problem(side, uplo, diag, m, n, seed=41, pad=3):
 
 
 @pytest.mark.parametrize("impl", ["strsm_blocked_inv64", "strsm_dataflow64", "strsm_panel256", "strsm_panel256_inv",
+                                  "strsm_panel256_fused",
                                   "strsm_recursive"])
 @pytest.mark.parametrize("v", VARIANTS, ids=vid)
 @pytest.mark.parametrize("shape", [(5, 3), (64, 64), (65, 17), (200, 130), (300, 77), (600, 40), (70, 520),
