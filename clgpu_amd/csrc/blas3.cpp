@@ -530,7 +530,7 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
     t.canonical_l(&ia.L, &ia.l_si, &ia.l_sk);
     ia.unit = t.unit ? 1 : 0;
     ia.conj = t.conj ? 1 : 0;
-    const float one[2] = {1.0f, 0.0f}, minus_one[2] = {-1.0f, 0.0f}, zero[2] = {0.0f, 0.0f};
+    const float one[2] = {1.0f, 0.0f}, minus_one[2] = {-1.0f, 0.0f};
     if (h->forced_impl == "ctrsm_recursive_inv64") {
         float* invD = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
             (size_t)nblk * kLeaf * kLeaf * 2 * sizeof(float), h->stream));

@@ -55,15 +55,12 @@ __device__ __forceinline__ bool tile_window(const GemmArgs& g, int m0, int n0, i
 {
     if (g.write_tri == ICL_TRI_LOWER && m0 + BM - 1 < n0) return false;
     if (g.write_tri == ICL_TRI_UPPER && m0 > n0 + BN - 1) return false;
-    kb = 0;
-    ke = g.K;
-    switch (g.kband) {
-    case ICL_KBAND_K_BELOW_M_END: ke = min(g.K, m0 + BM); break;
-    case ICL_KBAND_K_FROM_M0: kb = min(g.K, m0); break;
-    case ICL_KBAND_K_BELOW_N_END: ke = min(g.K, n0 + BN); break;
-    case ICL_KBAND_K_FROM_N0: kb = min(g.K, n0); break;
-    default: break;
-    }
+    // Both bounds as values (see tile_coords_level3 on stores per case).
+    const int kband = g.kband;
+    kb = kband == ICL_KBAND_K_FROM_M0 ? min(g.K, m0) : kband == ICL_KBAND_K_FROM_N0 ? min(g.K, n0) : 0;
+    ke = kband == ICL_KBAND_K_BELOW_M_END   ? min(g.K, m0 + BM)
+         : kband == ICL_KBAND_K_BELOW_N_END ? min(g.K, n0 + BN)
+                                            : g.K;
     return true;
 }
 
@@ -426,22 +423,42 @@ struct CSplitSlab {
 // ---------------------------------------------------------------------------
 // TRI_CHECK: honour g.write_tri (the FP32 kernels always test it; the uniform
 // branch also keeps the compiler from hoisting every store address of the tile
-// ahead of the stores, which otherwise spills registers).
+// ahead of the stores).
+// Register r of a lane holds column n = ncol0 + 4 (lane / 32) + dr(r) with the
+// compile-time dr(r) = 8 (r / 4) + r % 4, so every store address is the lane's
+// column-0 pointer plus the wave-uniform dr(r) * ldc: one 64-bit pointer per lane
+// is live across the tile instead of one address computation per register.
+__device__ __forceinline__ constexpr int mfma32_reg_row(int r) { return 8 * (r >> 2) + (r & 3); }
+
 template <bool EDGE, bool TRI_CHECK = false>
 __device__ __forceinline__ void store_real_tile(const GemmArgs& g, const f32x16& acc, int mrow, int ncol0, int lane)
 {
     // mrow: m index of this lane; ncol0: n index of register 0's 32x32 block origin.
     const TriMask tri(g.mask);
+    const int nl = ncol0 + 4 * (lane >> 5);
+    float* const c0 = g.C + (long)nl * g.ldc + mrow;
+    // beta == 0 (C not read) and beta != 0 as two straight-line loops under one
+    // uniform branch, rather than a branch around each load of C.
+    if (g.beta_re == 0.0f) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int n = ncol0 + mfma32_row(lane, r);
-        if (EDGE && (mrow >= g.M || n >= g.N)) continue;
-        if constexpr (TRI_CHECK)
-            if (!tri_writes(g.write_tri, mrow, n)) continue;
-        float* c = g.C + (long)n * g.ldc + mrow;
-        float v = g.alpha_re * acc[r];
-        if (g.beta_re != 0.0f) v = fmaf(g.beta_re, *c, v);
-        *c = tri.keep(mrow, n) ? v : 0.0f;
+        for (int r = 0; r < 16; ++r) {
+            const int dr = mfma32_reg_row(r), n = nl + dr;
+            if (EDGE && (mrow >= g.M || n >= g.N)) continue;
+            if constexpr (TRI_CHECK)
+                if (!tri_writes(g.write_tri, mrow, n)) continue;
+            c0[(long)dr * g.ldc] = tri.keep(mrow, n) ? g.alpha_re * acc[r] : 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int dr = mfma32_reg_row(r), n = nl + dr;
+            if (EDGE && (mrow >= g.M || n >= g.N)) continue;
+            if constexpr (TRI_CHECK)
+                if (!tri_writes(g.write_tri, mrow, n)) continue;
+            float* c = c0 + (long)dr * g.ldc;
+            const float v = fmaf(g.beta_re, *c, g.alpha_re * acc[r]);
+            *c = tri.keep(mrow, n) ? v : 0.0f;
+        }
     }
 }
 
@@ -454,13 +471,15 @@ __device__ __forceinline__ void store_complex_tile(const GemmArgs& g, const f32x
     const bool beta_zero = (g.beta_re == 0.0f && g.beta_im == 0.0f);
     const float conj_sign = (L3 && (g.cflags & ICL_CFLAG_CONJ_ACC)) ? -1.0f : 1.0f;
     const bool real_diag = L3 && (g.cflags & ICL_CFLAG_REAL_DIAG) != 0;
+    const int nl = ncol0 + 4 * (lane >> 5);
+    float* const c0 = g.C + 2 * ((long)nl * g.ldc + mrow);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const int n = ncol0 + mfma32_row(lane, r);
+        const int dr = mfma32_reg_row(r), n = nl + dr;
         if (EDGE && (mrow >= g.M || n >= g.N)) continue;
         if constexpr (L3)
             if (!tri_writes(g.write_tri, mrow, n)) continue;
-        float* c = g.C + 2 * ((long)n * g.ldc + mrow);
+        float* c = c0 + 2 * (long)dr * g.ldc;
         const float xr = accr[r], xi = conj_sign * acci[r];
         float vr = fmaf(g.alpha_re, xr, -g.alpha_im * xi);
         float vi = fmaf(g.alpha_re, xi, g.alpha_im * xr);

@@ -8,7 +8,7 @@ namespace icl {
 // Tiles of tile-row tm that meet the triangle a level-3 update writes
 // (write_tri 1 = ICL_TRI_LOWER: m >= n, 2 = ICL_TRI_UPPER: m <= n): columns
 // [lo, lo + cnt).
-__host__ __device__ inline void tri_tile_row(int write_tri, int tm, int tiles_n, int bm, int bn, int& lo, int& cnt)
+__host__ __device__ __attribute__((always_inline)) inline void tri_tile_row(int write_tri, int tm, int tiles_n, int bm, int bn, int& lo, int& cnt)
 {
     if (write_tri == 1) {
         lo = 0;

@@ -114,10 +114,11 @@ __device__ __forceinline__ bool tile_coords_level3(int write_tri, int kband, int
     int b = blockIdx.x;
     if (write_tri != 0) {
         if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-        for (tm = 0; tm < tiles_m; ++tm) {
+        for (int i = 0; i < tiles_m; ++i) {
             int lo, cnt;
-            tri_tile_row(write_tri, tm, tiles_n, BM, BN, lo, cnt);
+            tri_tile_row(write_tri, i, tiles_n, BM, BN, lo, cnt);
             if (b < cnt) {
+                tm = i;
                 tn = lo + b;
                 return true;
             }
@@ -126,12 +127,15 @@ __device__ __forceinline__ bool tile_coords_level3(int write_tri, int kband, int
         return false;
     }
     if (b >= tiles_m * tiles_n) return false;
-    switch (kband) {
-    case 1: tm = tiles_m - 1 - b / tiles_n; tn = b % tiles_n; break; // work grows with tm
-    case 2: tm = b / tiles_n; tn = b % tiles_n; break;               // work shrinks with tm
-    case 3: tn = tiles_n - 1 - b / tiles_m; tm = b % tiles_m; break; // work grows with tn
-    default: tn = b / tiles_m; tm = b % tiles_m; break;              // work shrinks with tn
-    }
+    // Values, not stores through tm / tn per case: stores to different
+    // references in different cases get merged into one store through a
+    // selected pointer, which keeps tm and tn in scratch memory.
+    const bool by_m = kband == 1 || kband == 2;
+    const int q = b / (by_m ? tiles_n : tiles_m), r = b % (by_m ? tiles_n : tiles_m);
+    const int m_idx = kband == 1 ? tiles_m - 1 - q : kband == 2 ? q : r; // work grows / shrinks with tm
+    const int n_idx = kband == 1 || kband == 2 ? r : kband == 3 ? tiles_n - 1 - q : q; // ... with tn
+    tm = m_idx;
+    tn = n_idx;
     return true;
 }
 

@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(WVM* WVN * 64) sgemm_f32_dma_kernel(GemmArgs g
     for (int i = 0; i < Cfg::FM; ++i)
 #pragma unroll
         for (int j = 0; j < Cfg::FN; ++j)
-            store_real_tile<false, true>(g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
+            store_real_tile<false, L3>(g, acc[i][j], m0 + wm0 + 32 * i + (lane & 31), n0 + wn0 + 32 * j, lane);
 }
 
 template <int BM, int BN, int WVM, int WVN, bool A_KC, bool B_KC>
