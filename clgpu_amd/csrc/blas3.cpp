@@ -142,7 +142,13 @@ static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, co
     void* ws = nullptr;
     if (v->workspace) {
         const size_t bytes = v->workspace(&g);
-        if (bytes) ws = h->scratch[iclblasContext::kSlotGemmWorkspace].get(bytes, h->stream);
+        if (bytes && h->ext_ws) {
+            if (bytes > h->ext_ws_bytes)
+                throw status_error(ICLBLAS_STATUS_INVALID_VALUE, "GEMM workspace exceeds the caller's scratch");
+            ws = h->ext_ws;
+        } else if (bytes) {
+            ws = h->scratch[iclblasContext::kSlotGemmWorkspace].get(bytes, h->stream);
+        }
     }
     // A variant with a prepass marks its main kernel itself (icl_timer_mark).
     if (!v->workspace) hip_check(icl_timer_mark(h->stream), "icl_timer_mark");

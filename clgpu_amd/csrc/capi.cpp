@@ -81,6 +81,12 @@ ICLBLAS_API iclblasStatus_t iclblasCreate(iclblasHandle_t* handle)
         return ICLBLAS_STATUS_NOT_INITIALIZED;
     }
     h->own_stream = true;
+    // The implementation name is copied on every call: room for the longest one
+    // now, so that calls do not allocate host memory for it.
+    h->last_impl.reserve(64);
+    // iclblasSgemmNoAlloc's static scratch on this device (once per device; a
+    // failure is reported by that entry point, not here).
+    (void)icl_noalloc_scratch_init(dev);
     *handle = h;
     return ICLBLAS_STATUS_SUCCESS;
 }
@@ -130,6 +136,38 @@ ICLBLAS_API iclblasStatus_t iclblasSgemm(iclblasHandle_t handle, iclblasOperatio
             stage_out<float>(h, C, dC, m, n, ldc);
         }
         finish(h);
+    });
+}
+
+ICLBLAS_API iclblasStatus_t iclblasSgemmNoAlloc(iclblasHandle_t handle, iclblasOperation_t transa,
+                                                iclblasOperation_t transb, int m, int n, int k, const float* alpha,
+                                                float* A, int lda, float* B, int ldb, const float* beta, float* C,
+                                                int ldc)
+{
+    bool work = false;
+    const iclblasStatus_t st = gemm_checks(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, &work);
+    if (!work) return st;
+    return guarded([&] {
+        HandleLock lock(handle);
+        iclblasContext* h = handle;
+        void* ws = icl_noalloc_scratch(h->device);
+        if (!ws) throw status_error(ICLBLAS_STATUS_NOT_INITIALIZED, "no static scratch on this device");
+        // The scratch replaces the handle's grow-only workspace for this call only.
+        struct ExtWorkspace {
+            iclblasContext* h;
+            ExtWorkspace(iclblasContext* hh, void* p, size_t bytes) : h(hh)
+            {
+                h->ext_ws = p;
+                h->ext_ws_bytes = bytes;
+            }
+            ~ExtWorkspace()
+            {
+                h->ext_ws = nullptr;
+                h->ext_ws_bytes = 0;
+            }
+        } guard(h, ws, ICLBLAS_NOALLOC_SCRATCH_BYTES);
+        icl::sgemm_dev(h, math_of(h), transa, transb, m, n, k, *alpha, A, lda, B, ldb, *beta, C, ldc);
+        icl::hip_check(hipGetLastError(), "launch");
     });
 }
 

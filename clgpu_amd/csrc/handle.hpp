@@ -206,6 +206,11 @@ struct iclblasContext {
     int* trsm_error = nullptr; // device word set by an aborted dataflow STRSM (allocated on first use)
     bool trsm_error_pending = false; // a dataflow STRSM was queued since the word was last read
     icl::KernelTimer timer;
+    // Caller-provided GEMM workspace (iclblasSgemmNoAlloc): while set, GEMMs
+    // place their workspace here instead of in the grow-only scratch, and a
+    // GEMM needing more than ext_ws_bytes fails with INVALID_VALUE.
+    void* ext_ws = nullptr;
+    size_t ext_ws_bytes = 0;
 
     static bool valid(const iclblasContext* h) { return h != nullptr && h->tag == kTag; }
 };

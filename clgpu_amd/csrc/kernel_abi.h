@@ -367,6 +367,14 @@ typedef struct icl_expand_args {
 hipError_t icl_expand_real(const icl_expand_args* a, hipStream_t stream);
 hipError_t icl_expand_complex(const icl_expand_args* a, hipStream_t stream);
 
+/* The library's statically allocated device scratch (ICLBLAS_NOALLOC_SCRATCH_BYTES
+ * per device, iclblasSgemmNoAlloc's workspace). icl_noalloc_scratch_init resolves
+ * its address on `device`, which must be current; once per device, later calls
+ * return the first result. icl_noalloc_scratch returns that address, or NULL
+ * before a successful init on that device. */
+hipError_t icl_noalloc_scratch_init(int device);
+void* icl_noalloc_scratch(int device);
+
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -104,7 +104,11 @@ _SIGS = {
     "iclblasGetImplementations": [ct.c_char_p, _I],
     "iclblasSetKernelTiming": [_P, _I],
     "iclblasGetKernelTiming": [_P, _FP, ct.POINTER(_I)],
+    "iclblasSgemmNoAlloc": [_P, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _FP, _P, _I],
 }
+
+# Bytes of the library's static device scratch (ICLBLAS_NOALLOC_SCRATCH_BYTES, iclBLAS_ext.h)
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 def implementation_names() -> list[str]:
@@ -179,6 +183,12 @@ def iclblasDestroy(handle):
 def iclblasSgemm(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
     return load().iclblasSgemm(handle, transa, transb, m, n, k, _fscalar(alpha), _ptr(A), lda, _ptr(B), ldb,
                                _fscalar(beta), _ptr(C), ldc)
+
+
+def iclblasSgemmNoAlloc(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    """SGEMM on device pointers that allocates nothing (iclBLAS_ext.h)."""
+    return load().iclblasSgemmNoAlloc(handle, transa, transb, m, n, k, _fscalar(alpha), _ptr(A), lda, _ptr(B), ldb,
+                                      _fscalar(beta), _ptr(C), ldc)
 
 
 def iclblasCgemm(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):

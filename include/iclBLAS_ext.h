@@ -118,6 +118,31 @@ ICLBLAS_API iclblasStatus_t iclblasStrsmBackward(iclblasHandle_t handle, iclblas
 ICLBLAS_API iclblasStatus_t iclblasSetKernelTiming(iclblasHandle_t handle, int enable);
 ICLBLAS_API iclblasStatus_t iclblasGetKernelTiming(iclblasHandle_t handle, float* total_ms, int* launches);
 
+/* Bytes of the library's statically allocated device scratch, per GPU. */
+#define ICLBLAS_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* SGEMM that allocates no memory. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, validation and
+ * result as iclblasSgemm, except that A, B and C are always device pointers
+ * (whatever the memory mode) and the call is queued on the handle's stream and
+ * returns without waiting. It calls no hipMalloc, hipFree or other allocating
+ * routine: in the split-BF16 math mode the packed operands go to a device
+ * array of ICLBLAS_NOALLOC_SCRATCH_BYTES that the library holds on each GPU
+ * from its first iclblasCreate there until the process ends. A call whose
+ * packed operands need more than that, (M + N) * K * 4 bytes with M and N
+ * rounded up to the kernel tile (256) and K to 32, returns
+ * ICLBLAS_STATUS_INVALID_VALUE and queues nothing; the FP32 math mode and
+ * products shallower than 256 use no scratch and have no such limit.
+ * ICLBLAS_STATUS_NOT_INITIALIZED: the scratch could not be set up on the
+ * handle's GPU. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other handles) race and must be ordered
+ * by the caller. */
+ICLBLAS_API iclblasStatus_t iclblasSgemmNoAlloc(iclblasHandle_t handle, iclblasOperation_t transa,
+                                                iclblasOperation_t transb, int m, int n, int k,
+                                                const float* alpha, float* A, int lda, float* B, int ldb,
+                                                const float* beta, float* C, int ldc);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
