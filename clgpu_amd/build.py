@@ -7,6 +7,12 @@ clgpu_amd/_build/.
 
     python -m clgpu_amd.build            # full rebuild
     python -m clgpu_amd.build --incremental
+    python -m clgpu_amd.build --sanitize # + clgpu_amd/_build/san/libiclBLAS.so
+
+The sanitizer build compiles the host layer (*.cpp) with g++ and
+AddressSanitizer + UndefinedBehaviorSanitizer (host code only; the kernels
+are the regular objects) into clgpu_amd/_build/san/libiclBLAS.so, which the
+CPU tests load with libasan preloaded (tests/test_sanitizers.py).
 """
 from __future__ import annotations
 
@@ -86,12 +92,55 @@ def build(incremental: bool = False, jobs: int | None = None, verbose: bool = Tr
     return LIB
 
 
+SAN = BUILD / "san"
+SAN_LIB = SAN / "libiclBLAS.so"
+SAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+             "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized(jobs: int | None = None, verbose: bool = True) -> Path:
+    """Host layer under ASan + UBSan (g++), linked with the regular kernel objects."""
+    kernel_objs = [BUILD / (p.name + ".o") for p in sorted(CSRC.glob("*.hip"))]
+    if not all(o.exists() for o in kernel_objs):
+        build(incremental=True, jobs=jobs, verbose=False)
+    SAN.mkdir(parents=True, exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    flags = SAN_FLAGS + ["-std=c++17", "-fPIC", "-fvisibility=hidden", "-DEXPORT_ICLBLAS_SYMBOLS",
+                         "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{INCLUDE}", f"-I{CSRC}"]
+
+    def one(src: Path) -> Path:
+        obj = SAN / (src.name + ".o")
+        tmp = SAN / (src.name + f".{os.getpid()}.o")
+        res = subprocess.run([cxx] + flags + ["-c", str(src), "-o", str(tmp)], capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"sanitizer compile failed: {src.name}\n{res.stdout}{res.stderr}")
+        os.replace(tmp, obj)
+        return obj
+
+    with cf.ThreadPoolExecutor(max_workers=jobs or min(8, os.cpu_count() or 4)) as ex:
+        host_objs = list(ex.map(one, sorted(CSRC.glob("*.cpp"))))
+    tmp = SAN_LIB.with_suffix(f".so.tmp{os.getpid()}")
+    cmd = [cxx, "-shared", "-fsanitize=address,undefined", "-o", str(tmp)] + [str(o) for o in host_objs] + \
+          [str(o) for o in kernel_objs] + [f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib",
+                                          "-Wl,--no-undefined", "-Wl,-soname,libiclBLAS.so"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"sanitizer link failed\n{' '.join(cmd)}\n{res.stdout}{res.stderr}")
+    os.replace(tmp, SAN_LIB)
+    if verbose:
+        print(f"built {SAN_LIB}")
+    return SAN_LIB
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--incremental", action="store_true")
     ap.add_argument("-j", "--jobs", type=int, default=None)
+    ap.add_argument("--sanitize", action="store_true", help="also build the ASan/UBSan host layer")
     a = ap.parse_args()
     build(incremental=a.incremental, jobs=a.jobs)
+    if a.sanitize:
+        build_sanitized(jobs=a.jobs)
 
 
 if __name__ == "__main__":

@@ -81,6 +81,7 @@ ICLBLAS_API iclblasStatus_t iclblasCreate(iclblasHandle_t* handle)
         return ICLBLAS_STATUS_NOT_INITIALIZED;
     }
     h->own_stream = true;
+    for (auto& s : h->scratch) s.set_budget(&h->budget);
     // The implementation name is copied on every call: room for the longest one
     // now, so that calls do not allocate host memory for it.
     h->last_impl.reserve(64);
@@ -105,10 +106,12 @@ ICLBLAS_API iclblasStatus_t iclblasDestroy(iclblasHandle_t handle)
     return ICLBLAS_STATUS_SUCCESS;
 }
 
-ICLBLAS_API iclblasStatus_t iclblasSgemm(iclblasHandle_t handle, iclblasOperation_t transa, iclblasOperation_t transb,
+ICLBLAS_API iclblasStatus_t iclblasSgemm(iclblasHandle_t handle, iclblasOperation_t transa_, iclblasOperation_t transb_,
                                          int m, int n, int k, const float* alpha, float* A, int lda, float* B, int ldb,
                                          const float* beta, float* C, int ldc)
 {
+    const int transa = enum_arg(transa_);
+    const int transb = enum_arg(transb_);
     bool work = false;
     const iclblasStatus_t st = gemm_checks(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, &work);
     if (!work) return st;
@@ -139,11 +142,13 @@ ICLBLAS_API iclblasStatus_t iclblasSgemm(iclblasHandle_t handle, iclblasOperatio
     });
 }
 
-ICLBLAS_API iclblasStatus_t iclblasSgemmNoAlloc(iclblasHandle_t handle, iclblasOperation_t transa,
-                                                iclblasOperation_t transb, int m, int n, int k, const float* alpha,
+ICLBLAS_API iclblasStatus_t iclblasSgemmNoAlloc(iclblasHandle_t handle, iclblasOperation_t transa_,
+                                                iclblasOperation_t transb_, int m, int n, int k, const float* alpha,
                                                 float* A, int lda, float* B, int ldb, const float* beta, float* C,
                                                 int ldc)
 {
+    const int transa = enum_arg(transa_);
+    const int transb = enum_arg(transb_);
     bool work = false;
     const iclblasStatus_t st = gemm_checks(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, &work);
     if (!work) return st;
@@ -171,10 +176,12 @@ ICLBLAS_API iclblasStatus_t iclblasSgemmNoAlloc(iclblasHandle_t handle, iclblasO
     });
 }
 
-ICLBLAS_API iclblasStatus_t iclblasCgemm(iclblasHandle_t handle, iclblasOperation_t transa, iclblasOperation_t transb,
+ICLBLAS_API iclblasStatus_t iclblasCgemm(iclblasHandle_t handle, iclblasOperation_t transa_, iclblasOperation_t transb_,
                                          int m, int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda,
                                          oclComplex_t* B, int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc)
 {
+    const int transa = enum_arg(transa_);
+    const int transb = enum_arg(transb_);
     bool work = false;
     const iclblasStatus_t st = gemm_checks(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, &work);
     if (!work) return st;
@@ -215,8 +222,8 @@ ICLBLAS_API iclblasStatus_t iclblasCgemm(iclblasHandle_t handle, iclblasOperatio
     });
 }
 
-static iclblasStatus_t strsm_common(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo,
-                                    iclblasOperation_t trans, iclblasDiagType_t diag, int m, int n, const float* alpha,
+static iclblasStatus_t strsm_common(iclblasHandle_t handle, int side, int uplo, int trans, int diag, int m, int n,
+                                    const float* alpha,
                                     float* A, int lda, float* B, int ldb, int sweeps)
 {
     bool work = false;
@@ -252,10 +259,14 @@ static iclblasStatus_t strsm_common(iclblasHandle_t handle, iclblasSideMode_t si
 
 // reference: libs/blas/shlib/iclBLAS_C.cpp:525-535 (m == 0 or n == 0 -> SUCCESS
 // first, then the STRSM checks).
-ICLBLAS_API iclblasStatus_t iclblasCtrsm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo,
-                                         iclblasOperation_t trans, iclblasDiagType_t diag, int m, int n,
+ICLBLAS_API iclblasStatus_t iclblasCtrsm(iclblasHandle_t handle, iclblasSideMode_t side_, iclblasFillMode_t uplo_,
+                                         iclblasOperation_t trans_, iclblasDiagType_t diag_, int m, int n,
                                          const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb)
 {
+    const int trans = enum_arg(trans_);
+    const int side = enum_arg(side_);
+    const int uplo = enum_arg(uplo_);
+    const int diag = enum_arg(diag_);
     bool work = false;
     const iclblasStatus_t st =
         trsm_checks(handle, side, uplo, trans, diag, m, n, reinterpret_cast<const float*>(alpha),
@@ -284,30 +295,39 @@ ICLBLAS_API iclblasStatus_t iclblasCtrsm(iclblasHandle_t handle, iclblasSideMode
     });
 }
 
-ICLBLAS_API iclblasStatus_t iclblasStrsm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo,
-                                         iclblasOperation_t trans, iclblasDiagType_t diag, int m, int n,
+ICLBLAS_API iclblasStatus_t iclblasStrsm(iclblasHandle_t handle, iclblasSideMode_t side_, iclblasFillMode_t uplo_,
+                                         iclblasOperation_t trans_, iclblasDiagType_t diag_, int m, int n,
                                          const float* alpha, float* A, int lda, float* B, int ldb)
 {
+    const int trans = enum_arg(trans_);
+    const int side = enum_arg(side_);
+    const int uplo = enum_arg(uplo_);
+    const int diag = enum_arg(diag_);
     return strsm_common(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, 0);
 }
 
-ICLBLAS_API iclblasStatus_t iclblasStrsmRefined(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo,
-                                                iclblasOperation_t trans, iclblasDiagType_t diag, int m, int n,
+ICLBLAS_API iclblasStatus_t iclblasStrsmRefined(iclblasHandle_t handle, iclblasSideMode_t side_, iclblasFillMode_t uplo_,
+                                                iclblasOperation_t trans_, iclblasDiagType_t diag_, int m, int n,
                                                 const float* alpha, float* A, int lda, float* B, int ldb,
                                                 int max_sweeps)
 {
+    const int trans = enum_arg(trans_);
+    const int side = enum_arg(side_);
+    const int uplo = enum_arg(uplo_);
+    const int diag = enum_arg(diag_);
     return strsm_common(handle, side, uplo, trans, diag, m, n, alpha, A, lda, B, ldb, max_sweeps);
 }
 
 // ---------------------------------------------------------------------------
 // Modes, stream, implementation override
 // ---------------------------------------------------------------------------
-ICLBLAS_API iclblasStatus_t iclblasSetMemoryMode(iclblasHandle_t handle, iclblasMemoryMode_t mode)
+ICLBLAS_API iclblasStatus_t iclblasSetMemoryMode(iclblasHandle_t handle, iclblasMemoryMode_t mode_)
 {
+    const int mode = enum_arg(mode_);
     if (!iclblasContext::valid(handle)) return ICLBLAS_STATUS_INVALID_VALUE;
     if (mode != ICLBLAS_MEMORY_HOST && mode != ICLBLAS_MEMORY_DEVICE) return ICLBLAS_STATUS_INVALID_VALUE;
     std::lock_guard<std::mutex> g(handle->mu);
-    handle->memory_mode = mode;
+    handle->memory_mode = static_cast<iclblasMemoryMode_t>(mode);
     return ICLBLAS_STATUS_SUCCESS;
 }
 
@@ -318,12 +338,13 @@ ICLBLAS_API iclblasStatus_t iclblasGetMemoryMode(iclblasHandle_t handle, iclblas
     return ICLBLAS_STATUS_SUCCESS;
 }
 
-ICLBLAS_API iclblasStatus_t iclblasSetMathMode(iclblasHandle_t handle, iclblasMathMode_t mode)
+ICLBLAS_API iclblasStatus_t iclblasSetMathMode(iclblasHandle_t handle, iclblasMathMode_t mode_)
 {
+    const int mode = enum_arg(mode_);
     if (!iclblasContext::valid(handle)) return ICLBLAS_STATUS_INVALID_VALUE;
     if (mode != ICLBLAS_MATH_FP32 && mode != ICLBLAS_MATH_SPLIT_BF16X3) return ICLBLAS_STATUS_INVALID_VALUE;
     std::lock_guard<std::mutex> g(handle->mu);
-    handle->math_mode = mode;
+    handle->math_mode = static_cast<iclblasMathMode_t>(mode);
     return ICLBLAS_STATUS_SUCCESS;
 }
 
@@ -451,14 +472,33 @@ ICLBLAS_API iclblasStatus_t iclblasGetKernelTiming(iclblasHandle_t handle, float
     });
 }
 
+ICLBLAS_API iclblasStatus_t iclblasSetScratchLimit(iclblasHandle_t handle, size_t bytes)
+{
+    if (!iclblasContext::valid(handle)) return ICLBLAS_STATUS_INVALID_VALUE;
+    std::lock_guard<std::mutex> g(handle->mu);
+    handle->budget.limit = bytes;
+    return ICLBLAS_STATUS_SUCCESS;
+}
+
+ICLBLAS_API iclblasStatus_t iclblasGetScratchUsage(iclblasHandle_t handle, size_t* held, size_t* limit)
+{
+    if (!iclblasContext::valid(handle) || !held || !limit) return ICLBLAS_STATUS_INVALID_VALUE;
+    std::lock_guard<std::mutex> g(handle->mu);
+    *held = handle->budget.held;
+    *limit = handle->budget.limit;
+    return ICLBLAS_STATUS_SUCCESS;
+}
+
 // ---------------------------------------------------------------------------
 // Adjoints
 // ---------------------------------------------------------------------------
-ICLBLAS_API iclblasStatus_t iclblasSgemmBackward(iclblasHandle_t handle, iclblasOperation_t transa,
-                                                 iclblasOperation_t transb, int m, int n, int k, const float* alpha,
+ICLBLAS_API iclblasStatus_t iclblasSgemmBackward(iclblasHandle_t handle, iclblasOperation_t transa_,
+                                                 iclblasOperation_t transb_, int m, int n, int k, const float* alpha,
                                                  float* A, int lda, float* B, int ldb, float* dC, int lddc, float* dA,
                                                  int ldda, float* dB, int lddb)
 {
+    const int transa = enum_arg(transa_);
+    const int transb = enum_arg(transb_);
     if (m < 0 || n < 0 || k < 0 || !alpha) return ICLBLAS_STATUS_INVALID_VALUE;
     if (!op_ok(transa) || !op_ok(transb)) return ICLBLAS_STATUS_INVALID_VALUE;
     const int ra = transa == ICLBLAS_OP_N ? m : k, ca = transa == ICLBLAS_OP_N ? k : m;
@@ -492,12 +532,16 @@ ICLBLAS_API iclblasStatus_t iclblasSgemmBackward(iclblasHandle_t handle, iclblas
     });
 }
 
-ICLBLAS_API iclblasStatus_t iclblasStrsmBackward(iclblasHandle_t handle, iclblasSideMode_t side,
-                                                 iclblasFillMode_t uplo, iclblasOperation_t trans,
-                                                 iclblasDiagType_t diag, int m, int n, const float* alpha, float* A,
+ICLBLAS_API iclblasStatus_t iclblasStrsmBackward(iclblasHandle_t handle, iclblasSideMode_t side_,
+                                                 iclblasFillMode_t uplo_, iclblasOperation_t trans_,
+                                                 iclblasDiagType_t diag_, int m, int n, const float* alpha, float* A,
                                                  int lda, float* X, int ldx, float* dX, int lddx, float* dA, int ldda,
                                                  float* dB, int lddb)
 {
+    const int trans = enum_arg(trans_);
+    const int side = enum_arg(side_);
+    const int uplo = enum_arg(uplo_);
+    const int diag = enum_arg(diag_);
     bool work = false;
     const iclblasStatus_t st = trsm_checks(handle, side, uplo, trans, diag, m, n, alpha, A, lda, X, ldx, &work);
     if (!work) return st;

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <new>
 
 #include "blas3.hpp"
@@ -30,6 +31,19 @@ iclblasStatus_t guarded(F&& f)
     } catch (...) {
         return ICLBLAS_STATUS_ERROR;
     }
+}
+
+// The int a caller passed for an enum parameter of the C API. A C caller may
+// pass any int there (the reference treats every non-zero op as transposed);
+// loading a value outside the enumerators' range through the C++ enum type is
+// undefined, so entry points read the parameter's representation instead.
+template <typename E>
+inline int enum_arg(const E& e)
+{
+    static_assert(sizeof(E) == sizeof(int), "C API enums are int-sized");
+    int v;
+    std::memcpy(&v, &e, sizeof v);
+    return v;
 }
 
 inline bool op_ok(int op) { return op == ICLBLAS_OP_N || op == ICLBLAS_OP_T || op == ICLBLAS_OP_C; }

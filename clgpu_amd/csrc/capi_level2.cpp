@@ -70,18 +70,20 @@ iclblasStatus_t gemv_common(iclblasHandle_t handle, bool cplx, int trans, int m,
 extern "C" {
 
 // reference: libs/blas/shlib/iclBLAS_S.cpp:474-485
-ICLBLAS_API iclblasStatus_t iclblasSgemv(iclblasHandle_t handle, iclblasOperation_t trans, int m, int n,
+ICLBLAS_API iclblasStatus_t iclblasSgemv(iclblasHandle_t handle, iclblasOperation_t trans_, int m, int n,
                                          const float* alpha, float* A, int lda, float* x, int incx, const float* beta,
                                          float* y, int incy)
 {
+    const int trans = enum_arg(trans_);
     return gemv_common(handle, false, trans, m, n, alpha, A, lda, x, incx, beta, y, incy);
 }
 
 // reference: libs/blas/shlib/iclBLAS_C.cpp:235-248
-ICLBLAS_API iclblasStatus_t iclblasCgemv(iclblasHandle_t handle, iclblasOperation_t trans, int m, int n,
+ICLBLAS_API iclblasStatus_t iclblasCgemv(iclblasHandle_t handle, iclblasOperation_t trans_, int m, int n,
                                          const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* x, int incx,
                                          const oclComplex_t* beta, oclComplex_t* y, int incy)
 {
+    const int trans = enum_arg(trans_);
     return gemv_common(handle, true, trans, m, n, reinterpret_cast<const float*>(alpha),
                        reinterpret_cast<const float*>(A), lda, reinterpret_cast<const float*>(x), incx,
                        reinterpret_cast<const float*>(beta), reinterpret_cast<float*>(y), incy);

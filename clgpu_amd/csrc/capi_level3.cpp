@@ -186,74 +186,94 @@ float* F(void* p) { return static_cast<float*>(p); }
 extern "C" {
 
 // reference: libs/blas/shlib/iclBLAS_S.cpp:387-397
-ICLBLAS_API iclblasStatus_t iclblasSsymm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, int m,
+ICLBLAS_API iclblasStatus_t iclblasSsymm(iclblasHandle_t handle, iclblasSideMode_t side_, iclblasFillMode_t uplo_, int m,
                                          int n, const float* alpha, float* A, int lda, float* B, int ldb,
                                          const float* beta, float* C, int ldc)
 {
+    const int side = enum_arg(side_);
+    const int uplo = enum_arg(uplo_);
     return symm_common(handle, false, false, side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 // reference: libs/blas/shlib/iclBLAS_S.cpp:399-408
-ICLBLAS_API iclblasStatus_t iclblasSsyrk(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans,
+ICLBLAS_API iclblasStatus_t iclblasSsyrk(iclblasHandle_t handle, iclblasFillMode_t uplo_, iclblasOperation_t trans_,
                                          int n, int k, const float* alpha, float* A, int lda, const float* beta,
                                          float* C, int ldc)
 {
+    const int trans = enum_arg(trans_);
+    const int uplo = enum_arg(uplo_);
     return sym_rank_front(handle, false, false, uplo, trans, n, k, alpha, A, lda, nullptr, 0, beta, C, ldc);
 }
 
 // reference: libs/blas/shlib/iclBLAS_S.cpp:410-420
-ICLBLAS_API iclblasStatus_t iclblasSsyr2k(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans,
+ICLBLAS_API iclblasStatus_t iclblasSsyr2k(iclblasHandle_t handle, iclblasFillMode_t uplo_, iclblasOperation_t trans_,
                                           int n, int k, const float* alpha, float* A, int lda, float* B, int ldb,
                                           const float* beta, float* C, int ldc)
 {
+    const int trans = enum_arg(trans_);
+    const int uplo = enum_arg(uplo_);
     return sym_rank_front(handle, false, true, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 // reference: libs/blas/shlib/iclBLAS_S.cpp:512-520
-ICLBLAS_API iclblasStatus_t iclblasStrmm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo,
-                                         iclblasOperation_t transa, iclblasDiagType_t diag, int m, int n,
+ICLBLAS_API iclblasStatus_t iclblasStrmm(iclblasHandle_t handle, iclblasSideMode_t side_, iclblasFillMode_t uplo_,
+                                         iclblasOperation_t transa_, iclblasDiagType_t diag_, int m, int n,
                                          const float* alpha, float* A, int lda, float* B, int ldb, float* C, int ldc)
 {
+    const int transa = enum_arg(transa_);
+    const int side = enum_arg(side_);
+    const int uplo = enum_arg(uplo_);
+    const int diag = enum_arg(diag_);
     return trmm_common(handle, false, side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, C, ldc);
 }
 
 // reference: libs/blas/shlib/iclBLAS_C.cpp:472-483
-ICLBLAS_API iclblasStatus_t iclblasCsymm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, int m,
+ICLBLAS_API iclblasStatus_t iclblasCsymm(iclblasHandle_t handle, iclblasSideMode_t side_, iclblasFillMode_t uplo_, int m,
                                          int n, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B,
                                          int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc)
 {
+    const int side = enum_arg(side_);
+    const int uplo = enum_arg(uplo_);
     return symm_common(handle, true, false, side, uplo, m, n, F(alpha), F(A), lda, F(B), ldb, F(beta), F(C), ldc);
 }
 
 // reference: libs/blas/shlib/iclBLAS_C.cpp:578-589
-ICLBLAS_API iclblasStatus_t iclblasChemm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo, int m,
+ICLBLAS_API iclblasStatus_t iclblasChemm(iclblasHandle_t handle, iclblasSideMode_t side_, iclblasFillMode_t uplo_, int m,
                                          int n, const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B,
                                          int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc)
 {
+    const int side = enum_arg(side_);
+    const int uplo = enum_arg(uplo_);
     return symm_common(handle, true, true, side, uplo, m, n, F(alpha), F(A), lda, F(B), ldb, F(beta), F(C), ldc);
 }
 
 // reference: libs/blas/shlib/iclBLAS_C.cpp:498-509
-ICLBLAS_API iclblasStatus_t iclblasCsyrk(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans,
+ICLBLAS_API iclblasStatus_t iclblasCsyrk(iclblasHandle_t handle, iclblasFillMode_t uplo_, iclblasOperation_t trans_,
                                          int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda,
                                          const oclComplex_t* beta, oclComplex_t* C, int ldc)
 {
+    const int trans = enum_arg(trans_);
+    const int uplo = enum_arg(uplo_);
     return sym_rank_front(handle, true, false, uplo, trans, n, k, F(alpha), F(A), lda, nullptr, 0, F(beta), F(C), ldc);
 }
 
 // reference: libs/blas/shlib/iclBLAS_C.cpp:485-496
-ICLBLAS_API iclblasStatus_t iclblasCsyr2k(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans,
+ICLBLAS_API iclblasStatus_t iclblasCsyr2k(iclblasHandle_t handle, iclblasFillMode_t uplo_, iclblasOperation_t trans_,
                                           int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda,
                                           oclComplex_t* B, int ldb, const oclComplex_t* beta, oclComplex_t* C, int ldc)
 {
+    const int trans = enum_arg(trans_);
+    const int uplo = enum_arg(uplo_);
     return sym_rank_front(handle, true, true, uplo, trans, n, k, F(alpha), F(A), lda, F(B), ldb, F(beta), F(C), ldc);
 }
 
 // reference: libs/blas/shlib/iclBLAS_C.cpp:537-548 (alpha and beta are real)
-ICLBLAS_API iclblasStatus_t iclblasCherk(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans,
+ICLBLAS_API iclblasStatus_t iclblasCherk(iclblasHandle_t handle, iclblasFillMode_t uplo_, iclblasOperation_t trans_,
                                          int n, int k, const float* alpha, oclComplex_t* A, int lda, const float* beta,
                                          oclComplex_t* C, int ldc)
 {
+    const int trans = enum_arg(trans_);
+    const int uplo = enum_arg(uplo_);
     if (n < 0 || k < 0) return ICLBLAS_STATUS_INVALID_VALUE;
     if (n == 0) return ICLBLAS_STATUS_SUCCESS;
     if (!alpha || !beta) return ICLBLAS_STATUS_INVALID_VALUE;
@@ -262,10 +282,12 @@ ICLBLAS_API iclblasStatus_t iclblasCherk(iclblasHandle_t handle, iclblasFillMode
 }
 
 // reference: libs/blas/shlib/iclBLAS_C.cpp:550-561 (beta is real)
-ICLBLAS_API iclblasStatus_t iclblasCher2k(iclblasHandle_t handle, iclblasFillMode_t uplo, iclblasOperation_t trans,
+ICLBLAS_API iclblasStatus_t iclblasCher2k(iclblasHandle_t handle, iclblasFillMode_t uplo_, iclblasOperation_t trans_,
                                           int n, int k, const oclComplex_t* alpha, oclComplex_t* A, int lda,
                                           oclComplex_t* B, int ldb, const float* beta, oclComplex_t* C, int ldc)
 {
+    const int trans = enum_arg(trans_);
+    const int uplo = enum_arg(uplo_);
     if (n < 0 || k < 0) return ICLBLAS_STATUS_INVALID_VALUE;
     if (n == 0) return ICLBLAS_STATUS_SUCCESS;
     if (!alpha || !beta) return ICLBLAS_STATUS_INVALID_VALUE;
@@ -276,11 +298,15 @@ ICLBLAS_API iclblasStatus_t iclblasCher2k(iclblasHandle_t handle, iclblasFillMod
 }
 
 // reference: libs/blas/shlib/iclBLAS_C.cpp:564-575
-ICLBLAS_API iclblasStatus_t iclblasCtrmm(iclblasHandle_t handle, iclblasSideMode_t side, iclblasFillMode_t uplo,
-                                         iclblasOperation_t transa, iclblasDiagType_t diag, int m, int n,
+ICLBLAS_API iclblasStatus_t iclblasCtrmm(iclblasHandle_t handle, iclblasSideMode_t side_, iclblasFillMode_t uplo_,
+                                         iclblasOperation_t transa_, iclblasDiagType_t diag_, int m, int n,
                                          const oclComplex_t* alpha, oclComplex_t* A, int lda, oclComplex_t* B, int ldb,
                                          oclComplex_t* C, int ldc)
 {
+    const int transa = enum_arg(transa_);
+    const int side = enum_arg(side_);
+    const int uplo = enum_arg(uplo_);
+    const int diag = enum_arg(diag_);
     if (n <= 0 || m <= 0 || !uplo_ok(uplo)) return ICLBLAS_STATUS_SUCCESS;
     return trmm_common(handle, true, side, uplo, transa, diag, m, n, F(alpha), F(A), lda, F(B), ldb, F(C), ldc);
 }

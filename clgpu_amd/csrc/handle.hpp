@@ -34,6 +34,13 @@ inline void hip_check(hipError_t e, const char* what)
     throw status_error(ICLBLAS_STATUS_EXECUTION_FAILED, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Device memory held by a handle's scratch buffers, and its cap
+// (iclblasSetScratchLimit; 0 = none).
+struct ScratchBudget {
+    size_t limit = 0;
+    size_t held = 0;
+};
+
 // A grow-only device allocation owned by the handle.
 class DeviceScratch {
 public:
@@ -42,24 +49,32 @@ public:
     DeviceScratch& operator=(const DeviceScratch&) = delete;
     ~DeviceScratch() { release(); }
 
+    void set_budget(ScratchBudget* b) { budget_ = b; }
+
     // Returns at least `bytes` of device memory. Growing waits for `stream` first
-    // because queued work may still use the old allocation.
+    // because queued work may still use the old allocation. Growing beyond the
+    // handle's cap fails like an exhausted device (ALLOC_FAILED), before anything
+    // is freed or allocated.
     void* get(size_t bytes, hipStream_t stream)
     {
         if (bytes <= cap_ && ptr_) return ptr_;
+        const size_t want = bytes < 256 ? 256 : bytes;
+        if (budget_ && budget_->limit && budget_->held - cap_ + want > budget_->limit)
+            throw status_error(ICLBLAS_STATUS_ALLOC_FAILED, "scratch limit of the handle");
         if (ptr_) {
             hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
             release();
         }
-        const size_t want = bytes < 256 ? 256 : bytes;
         hip_check(hipMalloc(&ptr_, want), "hipMalloc");
         cap_ = want;
+        if (budget_) budget_->held += want;
         return ptr_;
     }
 
     void release()
     {
         if (ptr_) (void)hipFree(ptr_);
+        if (budget_) budget_->held -= cap_;
         ptr_ = nullptr;
         cap_ = 0;
     }
@@ -67,6 +82,7 @@ public:
 private:
     void* ptr_ = nullptr;
     size_t cap_ = 0;
+    ScratchBudget* budget_ = nullptr;
 };
 
 // Host <-> device copies of the host-pointer memory mode through pinned
@@ -200,6 +216,7 @@ struct iclblasContext {
     std::string last_impl;
     int last_sweeps = 0; // refinement sweeps the last iclblasStrsmRefined ran
     std::mutex mu; // a handle serializes its own calls
+    icl::ScratchBudget budget; // cap and total of the scratch buffers below
     icl::DeviceScratch scratch[kScratchSlots];
     icl::PinnedStager stager;
     icl::CopyLane copy;

@@ -24,7 +24,9 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libiclBLAS.so"
+# ICLBLAS_LIBRARY selects another build of the same library (the sanitizer build
+# of the host layer, clgpu_amd/_build/san/libiclBLAS.so, runs the CPU tests).
+LIB_PATH = Path(os.environ.get("ICLBLAS_LIBRARY") or Path(__file__).resolve().parent / "libiclBLAS.so")
 
 # iclblasStatus_t (iclBLAS.h:104-117)
 STATUS_SUCCESS = 0
@@ -105,6 +107,8 @@ _SIGS = {
     "iclblasSetKernelTiming": [_P, _I],
     "iclblasGetKernelTiming": [_P, _FP, ct.POINTER(_I)],
     "iclblasSgemmNoAlloc": [_P, _I, _I, _I, _I, _I, _FP, _P, _I, _P, _I, _FP, _P, _I],
+    "iclblasSetScratchLimit": [_P, ct.c_size_t],
+    "iclblasGetScratchUsage": [_P, ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t)],
 }
 
 # Bytes of the library's static device scratch (ICLBLAS_NOALLOC_SCRATCH_BYTES, iclBLAS_ext.h)
@@ -346,6 +350,16 @@ class Handle:
 
     def set_kernel_timing(self, on: bool):
         check(load().iclblasSetKernelTiming(self.h, 1 if on else 0), "iclblasSetKernelTiming")
+
+    def set_scratch_limit(self, nbytes: int):
+        """Cap of the handle's device buffers in bytes (0 = none); beyond it calls fail with ALLOC_FAILED."""
+        check(load().iclblasSetScratchLimit(self.h, nbytes), "iclblasSetScratchLimit")
+
+    def scratch_usage(self) -> tuple[int, int]:
+        """(bytes held by the handle's device buffers, cap)."""
+        held, limit = ct.c_size_t(), ct.c_size_t()
+        check(load().iclblasGetScratchUsage(self.h, ct.byref(held), ct.byref(limit)), "iclblasGetScratchUsage")
+        return held.value, limit.value
 
     def kernel_timing(self) -> tuple[float, int]:
         """(summed ms, launches) of the main GEMM kernels timed since the last read."""

@@ -24,6 +24,8 @@
  *    tests/blas_funcs/test_helpers.hpp:78-102).
  */
 #pragma once
+#include <stddef.h>
+
 #include "iclBLAS.h"
 
 #ifdef __cplusplus
@@ -117,6 +119,17 @@ ICLBLAS_API iclblasStatus_t iclblasStrsmBackward(iclblasHandle_t handle, iclblas
  * launches are timed between two reads. */
 ICLBLAS_API iclblasStatus_t iclblasSetKernelTiming(iclblasHandle_t handle, int enable);
 ICLBLAS_API iclblasStatus_t iclblasGetKernelTiming(iclblasHandle_t handle, float* total_ms, int* launches);
+
+/* Caps the device memory held by the handle's own buffers (the grow-only
+ * scratch that stages host matrices in the host-pointer mode and holds the
+ * workspaces: packed split-BF16 operands, TRSM and refinement temporaries,
+ * level-3 expansions, GEMV partial sums), in bytes; 0 (the default) removes
+ * the cap. A call that would grow them beyond it returns
+ * ICLBLAS_STATUS_ALLOC_FAILED, as when the device runs out of memory; buffers
+ * already held are kept. Lowering the cap frees nothing. */
+ICLBLAS_API iclblasStatus_t iclblasSetScratchLimit(iclblasHandle_t handle, size_t bytes);
+/* Bytes the handle's buffers hold now, and the cap (0 = none). */
+ICLBLAS_API iclblasStatus_t iclblasGetScratchUsage(iclblasHandle_t handle, size_t* held, size_t* limit);
 
 /* Bytes of the library's statically allocated device scratch, per GPU. */
 #define ICLBLAS_NOALLOC_SCRATCH_BYTES (64u << 20)

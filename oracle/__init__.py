@@ -27,7 +27,9 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "liboracle.so"
+# ICLBLAS_ORACLE_LIBRARY selects another build of the oracle (oracle/_san, the
+# sanitizer build: `make -C oracle san`).
+LIB = Path(os.environ.get("ICLBLAS_ORACLE_LIBRARY") or HERE / "liboracle.so")
 REF_LIB = HERE / "_ref" / "libref_gemm.so"
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="F_CONTIGUOUS")

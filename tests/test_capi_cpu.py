@@ -15,7 +15,7 @@ from clgpu_amd import iclblas as ib
 
 def test_library_loads_and_exports_every_declared_symbol():
     declared = ib.declared_symbols()
-    assert len(declared) == 78 + 17  # reference API + extensions
+    assert len(declared) == 78 + 19  # reference API + extensions
     assert ib.exported_symbols() == declared
     out = subprocess.run(["nm", "-D", "--defined-only", str(ib.LIB_PATH)], capture_output=True, text=True,
                          check=True).stdout
