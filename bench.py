@@ -682,8 +682,9 @@ def sgemm_line(a, h, ib, stream, dev):
     head = "bf16x3" if "bf16x3" in results else "fp32"
     hr = results[head]
     # HBM bytes per launch come from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    # recorded in profiles/hbm_traffic.json (kernel name, workload, command); a
-    # live run has no counters, so the value is the recorded one for this kernel.
+    # summarised in profiles/sgemm_8192_hbm_traffic.md and recorded per kernel in
+    # profiles/hbm_traffic.json (a live run has no counters); the value is used
+    # only when the record names this kernel variant and this workload.
     traffic_rec = {}
     tf = ROOT / "profiles" / "hbm_traffic.json"
     if tf.exists():
@@ -716,7 +717,8 @@ def sgemm_line(a, h, ib, stream, dev):
                      "kernel": hr["implementation"], "kernel_ms": hr["kernel_ms"],
                      "note": "achieved = executed MFMA flops per launch (2mnk; 3 x 2mnk for split-BF16) / mean "
                              "duration of the main GEMM kernel (HIP events around it, same timed region); traffic "
-                             "= HBM bytes per launch from profiles/hbm_traffic.json"},
+                             "= HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes of "
+                             "profiles/sgemm_8192_hbm_traffic.md (profiles/hbm_traffic.json)"},
     }
     if "fp32" in results:
         r = results["fp32"]
