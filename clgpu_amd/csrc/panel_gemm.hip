@@ -2,21 +2,32 @@
 // blocked TRSMs: the panel step [X_p; B_r] = [W_p; -M_p] B_p (+ [0; B_r]) and the
 // batched M_p = L_rp W_p that precedes the steps.
 //
-// A GEMM of depth 256 is too short for a slab pipeline: with one 16-deep slab
-// in flight, a 64 x 64 tile waits out a global-memory latency per slab and the
-// launch takes 10-30 us whatever its size. Here a workgroup owns a 64 x 64 tile
-// of D and its four waves each take a 64-deep slice of the depth; a wave issues
-// every load of its slice at once, straight into the MFMA operand registers
-// (one wait per tile), runs 128 v_mfma_f32_32x32x2_f32, and the four partial
-// tiles are added through LDS in the epilogue. Two workgroups fit on a CU.
+// A GEMM of depth 256 is too short for the big GEMM kernels' tiles: at 128 x 128
+// or 256 x 256 a 4096 x 1024 step has 256 or 64 tiles, a fraction of the GPU,
+// and their slab pipelines barely fill before the K loop ends. Two kernels:
 //
-// Operand fragments: lane l (c = l % 32, h = l / 32) supplies, for MFMA s of
-// k-group q, the value at k = 64 w + 8 q + 4 h + s -- the same permutation of
-// the summation index for both operands -- so an operand whose k is contiguous
-// is read as one 16-byte load per (q, 32-block), and an operand whose other
-// dimension is contiguous as 4-byte loads coalesced over c.
-// The MFMA's lane dimension is the contiguous dimension of D, so the partial
-// tiles are written to LDS and D is written as 256-byte rows.
+// panel_gemm_dma_kernel (used whenever the operands allow 16-byte copies): a
+// workgroup owns a 64 x 64 tile of D and each of its four waves a 32 x 32 block
+// over the full depth. The depth runs in eight 32-deep chunks copied global ->
+// LDS with global_load_lds_dwordx4 (no VGPR staging), one chunk ahead of the
+// MFMAs in a two-stage ring of 32 KB, so four or five workgroups fit on a CU.
+// An operand whose rows (i for P, l for Q) are contiguous is stored S[k][64 rows]
+// (one 1 KB copy per four k rows); one whose k is contiguous S[row][32 k], 128-byte
+// rows of eight 16-byte chunks with logical chunk c of row r in slot c ^ ((r >> 1)
+// & 7), so the 16-byte fragment reads of 16 consecutive rows are free of bank
+// conflicts. Within a chunk, MFMA j of a wave takes k = 4 (2 (j / 4) + h) + j % 4
+// for lane half h in both operands (a common permutation of the summation index),
+// so one 16-byte read of a k-contiguous operand feeds four MFMAs. The MFMAs of a
+// wave alternate between two accumulators, added before the store. D's old
+// values (beta != 0) are loaded under the last chunk's MFMAs.
+//
+// panel_gemm_kernel (the fallback for unaligned operands): the four waves of a
+// workgroup each take a 64-deep slice of the depth for the whole 64 x 64 tile,
+// load it straight into the MFMA operand registers (one wait per tile) and add
+// their partial tiles through LDS.
+//
+// The MFMA's lane dimension is the contiguous dimension of D, so D is written as
+// 128-byte (DMA kernel) or 256-byte (fallback) column pieces.
 #include "kernel_abi.h"
 #include "mfma.hpp"
 
@@ -141,11 +152,143 @@ __global__ void __launch_bounds__(256, 2) panel_gemm_kernel(icl_panel_gemm_args 
     }
 }
 
+// ---- LDS-DMA kernel ----
+typedef __attribute__((address_space(1))) void pg_global_void;
+typedef __attribute__((address_space(3))) void pg_lds_void;
+
+constexpr int kPdKC = 32;                       // depth per chunk
+constexpr int kPdChunks = 256 / kPdKC;
+constexpr int kPdRowB = kPdKC * 4;              // bytes per row of a k-contiguous chunk
+constexpr int kPdOpB = kPgT * kPdKC * 4;        // bytes of one operand chunk (8 KB)
+constexpr int kPdStage = 2 * kPdOpB;            // P then Q
+constexpr int kPdLds = 2 * kPdStage;            // two stages: 32 KB
+constexpr int kPdBlocksPerWave = kPdOpB / 1024 / 4;
+
+__device__ __forceinline__ int pd_swizzle(int r) { return (r >> 1) & 7; }
+
+// Copies the 64-row x 32-deep chunk of an operand X(row, k) = X[row * rs + k * ks]
+// (rs == 1 when ROWC, else ks == 1) at rows row0.., depth k0.. into S. Rows at or
+// beyond `rows` read a valid row instead (their products are never stored); a
+// ROWC copy moves four rows, so there rows % 4 == 0 (checked by the host).
+template <bool ROWC>
+__device__ __forceinline__ void pd_issue(const float* X, long rs, long ks, int row0, int k0, int rows,
+                                         unsigned char* S, int wave, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < kPdBlocksPerWave; ++i) {
+        const int blk = wave + 4 * i; // 1 KB block of the chunk
+        const float* g;
+        if constexpr (ROWC) {
+            const int kr = 4 * blk + (lane >> 4); // k row; lane covers rows 4 (lane % 16) .. + 3
+            const int r = min(row0 + 4 * (lane & 15), rows - 4);
+            g = X + (long)(k0 + kr) * ks + r;
+        } else {
+            const int r = 8 * blk + (lane >> 3);            // row of the chunk
+            const int c = (lane & 7) ^ pd_swizzle(r);       // logical 16-byte chunk in this slot
+            g = X + (long)min(row0 + r, rows - 1) * rs + k0 + 4 * c;
+        }
+        __builtin_amdgcn_global_load_lds((pg_global_void*)g, (pg_lds_void*)(S + blk * 1024), 16, 0, 0);
+    }
+}
+
+// f[t] = X(rb + lane % 32, k) for MFMA 4 g + t of the chunk, k = 4 (2 g + h) + t.
+template <bool ROWC>
+__device__ __forceinline__ void pd_frag(const unsigned char* S, int rb, int g, int lane, float (&f)[4])
+{
+    const int h = lane >> 5, r = rb + (lane & 31), q = 2 * g + h;
+    if constexpr (ROWC) {
+        const float* s = reinterpret_cast<const float*>(S);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) f[t] = s[(4 * q + t) * kPgT + r];
+    } else {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(S + r * kPdRowB + ((q ^ pd_swizzle(r)) << 4));
+#pragma unroll
+        for (int t = 0; t < 4; ++t) f[t] = v[t];
+    }
+}
+
+template <bool P_I, bool Q_K, bool D_I, bool EDGE>
+__global__ void __launch_bounds__(256, 2) panel_gemm_dma_kernel(icl_panel_gemm_args a)
+{
+    extern __shared__ __attribute__((aligned(1024))) unsigned char pd_lds[];
+    const int b = blockIdx.x;
+    int pi = 0;
+    for (int i = 1; i < a.nprob; ++i)
+        if (b >= a.prob[i].tile0) pi = i;
+    const icl_panel_gemm_problem& pr = a.prob[pi];
+    const int rows = pr.rows, lines = pr.lines;
+    const int tiles_i = (rows + kPgT - 1) / kPgT;
+    const int t = b - pr.tile0;
+    const int i0 = (t % tiles_i) * kPgT, l0 = (t / tiles_i) * kPgT;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wi = 32 * (wave & 1), wl = 32 * (wave >> 1);
+
+    auto issue = [&](int c, int s) {
+        unsigned char* st = pd_lds + s * kPdStage;
+        pd_issue<P_I>(pr.P, pr.p_is, pr.p_ks, i0, c * kPdKC, EDGE ? rows : 1 << 30, st, wave, lane);
+        pd_issue<!Q_K>(pr.Q, pr.q_ls, pr.q_ks, l0, c * kPdKC, EDGE ? lines : 1 << 30, st + kPdOpB, wave, lane);
+    };
+
+    // Element e of a lane's 32 x 32 block: D(i, l) at (lane % 32, row e) or the other way round.
+    auto d_index = [&](int e, int& i, int& l) {
+        const int u = icl::mfma32_row(lane, e), v = lane & 31;
+        i = i0 + wi + (D_I ? v : u);
+        l = l0 + wl + (D_I ? u : v);
+    };
+    const float alpha = pr.alpha, beta = pr.beta;
+    float dold[16];
+
+    f32x16 acc[2] = {f32x16{}, f32x16{}};
+    issue(0, 0);
+#pragma unroll
+    for (int c = 0; c < kPdChunks; ++c) {
+        // chunk c in LDS for every wave, and every wave done with chunk c - 1
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (c + 1 < kPdChunks) issue(c + 1, (c + 1) & 1);
+        if (c + 1 == kPdChunks && beta != 0.0f) {
+            // D's old values are loaded under the last chunk's MFMAs rather than after them
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                int i, l;
+                d_index(e, i, l);
+                dold[e] = (!EDGE || (i < rows && l < lines)) ? pr.D[(long)i * pr.d_is + (long)l * pr.d_ls] : 0.0f;
+            }
+        }
+        const unsigned char* Ps = pd_lds + (c & 1) * kPdStage;
+        const unsigned char* Qs = Ps + kPdOpB;
+#pragma unroll
+        for (int g = 0; g < kPdKC / 8; ++g) {
+            float pf[4], qf[4];
+            pd_frag<P_I>(Ps, wi, g, lane, pf);
+            pd_frag<!Q_K>(Qs, wl, g, lane, qf);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                f32x16& ac = acc[s & 1];
+                // lanes along D's contiguous dimension: the b operand supplies it
+                ac = D_I ? icl::mfma_32x32x2_f32(qf[s], pf[s], ac) : icl::mfma_32x32x2_f32(pf[s], qf[s], ac);
+            }
+        }
+    }
+    const f32x16 sum = acc[0] + acc[1];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        int i, l;
+        d_index(e, i, l);
+        if (EDGE && (i >= rows || l >= lines)) continue;
+        float x = alpha * sum[e];
+        if (beta != 0.0f) x = fmaf(beta, dold[e], x);
+        pr.D[(long)i * pr.d_is + (long)l * pr.d_ls] = x;
+    }
+}
+
 using PgKern = void (*)(icl_panel_gemm_args);
 
 template <bool P_I, bool Q_K, bool D_I>
-PgKern pick(bool edge, bool vec)
+PgKern pick(bool edge, bool vec, bool dma)
 {
+    if (dma) return edge ? panel_gemm_dma_kernel<P_I, Q_K, D_I, true> : panel_gemm_dma_kernel<P_I, Q_K, D_I, false>;
     if (edge) return vec ? panel_gemm_kernel<P_I, Q_K, D_I, true, true> : panel_gemm_kernel<P_I, Q_K, D_I, true, false>;
     return vec ? panel_gemm_kernel<P_I, Q_K, D_I, false, true> : panel_gemm_kernel<P_I, Q_K, D_I, false, false>;
 }
@@ -162,6 +305,10 @@ extern "C" hipError_t icl_panel_gemm(icl_panel_gemm_args* a, hipStream_t stream)
     const bool p_i = f.p_is == 1, q_k = f.q_ks == 1, d_i = f.d_is == 1;
     long tiles = 0;
     bool edge = false, vec = true;
+    // The DMA kernel copies 16-byte pieces: every operand 16-byte aligned with its
+    // non-unit stride a multiple of 4 floats, and the row count of an operand whose
+    // rows are contiguous a multiple of 4.
+    bool dma = true;
     for (int i = 0; i < a->nprob; ++i) {
         icl_panel_gemm_problem& p = a->prob[i];
         if (p.rows < 0 || p.lines < 0) return hipErrorInvalidValue;
@@ -174,15 +321,17 @@ extern "C" hipError_t icl_panel_gemm(icl_panel_gemm_args* a, hipStream_t stream)
         edge = edge || p.rows % kPgT != 0 || p.lines % kPgT != 0;
         if (!p_i) vec = vec && aligned16(p.P, p.p_is);
         if (q_k) vec = vec && aligned16(p.Q, p.q_ls);
+        dma = dma && aligned16(p.P, p_i ? p.p_ks : p.p_is) && aligned16(p.Q, q_k ? p.q_ls : p.q_ks) &&
+              (!p_i || p.rows % 4 == 0) && (q_k || p.lines % 4 == 0);
     }
     if (tiles == 0) return hipSuccess;
     if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
     PgKern k = nullptr;
-    if (p_i && q_k && d_i) k = pick<true, true, true>(edge, vec);          // left steps; M_p of op N / right op N
-    else if (!p_i && !q_k && !d_i) k = pick<false, false, false>(edge, vec); // right steps
-    else if (!p_i && q_k && d_i) k = pick<false, true, true>(edge, vec);   // M_p, left op T
-    else if (p_i && !q_k && d_i) k = pick<true, false, true>(edge, vec);   // N_p, right op T
+    if (p_i && q_k && d_i) k = pick<true, true, true>(edge, vec, dma);          // left steps; M_p of op N / right op N
+    else if (!p_i && !q_k && !d_i) k = pick<false, false, false>(edge, vec, dma); // right steps
+    else if (!p_i && q_k && d_i) k = pick<false, true, true>(edge, vec, dma);   // M_p, left op T
+    else if (p_i && !q_k && d_i) k = pick<true, false, true>(edge, vec, dma);   // N_p, right op T
     else return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k, dim3((unsigned)tiles), dim3(256), 0, stream, *a);
+    hipLaunchKernelGGL(k, dim3((unsigned)tiles), dim3(256), dma ? kPdLds : 0, stream, *a);
     return hipGetLastError();
 }
