@@ -379,6 +379,209 @@ __global__ void __launch_bounds__(256, 3) residual_f64_kernel(icl_residual_args 
 }
 
 // ---------------------------------------------------------------------------
+// The same residual with its slabs copied global -> LDS by DMA
+// (global_load_lds_dwordx4): no VGPR staging, the next slab in flight while the
+// current one is multiplied, the triangle rule applied at the fragment read of
+// the slabs that cross the diagonal, and B0 loaded under the last slab's MFMAs.
+// For problems without edges (M, N multiples of 64, K of 32) whose operands
+// allow 16-byte copies; residual_f64_kernel runs the others.
+// LDS layouts of a 64-row x 32-deep FP32 slab (rows: m for P, n for Q):
+//  * rows contiguous: S[k][64], the rows of k row kr rotated by 16 ((kr >> 2) & 3)
+//    positions, so the four k rows one fragment read touches (k = kk + 4 g + s,
+//    g = lane / 16) sit on disjoint banks;
+//  * k contiguous: S[row][32], 128-byte rows of eight 16-byte chunks, logical
+//    chunk c of row r in slot c ^ ((r >> 1) & 7); a 16-byte read gives the four
+//    k of one lane's MFMAs s = 0..3.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(1))) void rd_global_void;
+typedef __attribute__((address_space(3))) void rd_lds_void;
+constexpr int kRdOpB = RT * RK * 4; // 8 KB per operand slab
+constexpr int kRdStage = 2 * kRdOpB;
+constexpr int kRdLds = 2 * kRdStage; // two stages: 32 KB
+
+__device__ __forceinline__ int rd_rot(int kr) { return 16 * ((kr >> 2) & 3); }
+
+// X(row, k) = X[row * rs + k * ks], rs == 1 when ROWC, else ks == 1.
+template <bool ROWC>
+__device__ __forceinline__ void rd_issue(const float* X, long rs, long ks, int row0, int k0, unsigned char* S,
+                                         int wave, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < kRdOpB / 1024 / 4; ++i) {
+        const int blk = wave + 4 * i;
+        const float* g;
+        if constexpr (ROWC) {
+            const int kr = 4 * blk + (lane >> 4);                  // k row
+            const int r = (4 * (lane & 15) - rd_rot(kr)) & (RT - 1); // rows r..r+3 go to slot lane % 16
+            g = X + (long)(k0 + kr) * ks + row0 + r;
+        } else {
+            const int r = 8 * blk + (lane >> 3);
+            const int c = (lane & 7) ^ ((r >> 1) & 7);
+            g = X + (long)(row0 + r) * rs + k0 + 4 * c;
+        }
+        __builtin_amdgcn_global_load_lds((rd_global_void*)g, (rd_lds_void*)(S + blk * 1024), 16, 0, 0);
+    }
+}
+
+// v[s] = X(x, kk + 4 g + s) for s = 0..3, g = lane / 16.
+template <bool ROWC>
+__device__ __forceinline__ void rd_frag(const unsigned char* S, int x, int kk, int lane, float (&v)[4])
+{
+    const int g = lane >> 4;
+    if constexpr (ROWC) {
+        const float* f = reinterpret_cast<const float*>(S);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int kr = kk + 4 * g + s;
+            v[s] = f[kr * RT + ((x + rd_rot(kr)) & (RT - 1))];
+        }
+    } else {
+        const f32x4 t = *reinterpret_cast<const f32x4*>(S + x * (RK * 4) + (((kk / 4 + g) ^ ((x >> 1) & 7)) << 4));
+#pragma unroll
+        for (int s = 0; s < 4; ++s) v[s] = t[s];
+    }
+}
+
+template <bool PT, bool QT>
+__global__ void __launch_bounds__(256, 2) residual_f64_dma_kernel(icl_residual_args a)
+{
+    extern __shared__ __attribute__((aligned(1024))) unsigned char rd_lds[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int tiles_m = a.M / RT, tiles_n = a.N / RT;
+    const int bid = blockIdx.x;
+    int tm[2], tn[2], ntiles = 1;
+    if (a.tri_on == 1) {
+        tn[0] = tn[1] = bid % tiles_n;
+        tm[0] = bid / tiles_n;
+        tm[1] = tiles_m - 1 - tm[0];
+        ntiles = tm[1] > tm[0] ? 2 : 1;
+    } else if (a.tri_on == 2) {
+        tm[0] = tm[1] = bid % tiles_m;
+        tn[0] = bid / tiles_m;
+        tn[1] = tiles_n - 1 - tn[0];
+        ntiles = tn[1] > tn[0] ? 2 : 1;
+    } else {
+        tm[0] = tm[1] = bid % tiles_m;
+        tn[0] = tn[1] = bid / tiles_m;
+    }
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    // P(i, k): P[i * ldp + k] when PT (k contiguous), else P[i + k * ldp] (rows
+    // contiguous); Q(k, c): Q[k * ldq + c] when QT (rows contiguous), else Q[c * ldq + k].
+    const long p_si = PT ? a.ldp : 1, p_sk = PT ? 1 : a.ldp;
+    const long q_si = QT ? 1 : a.ldq, q_sk = QT ? a.ldq : 1;
+
+    for (int t = 0; t < ntiles; ++t) {
+        const int m0 = tm[t] * RT, n0 = tn[t] * RT;
+        int k_lo = 0, k_hi = a.K;
+        if (a.tri_on == 1) {
+            if (a.tri_lower) k_hi = min(a.K, m0 + RT);
+            else k_lo = m0;
+        } else if (a.tri_on == 2) {
+            if (a.tri_lower) k_lo = n0;
+            else k_hi = min(a.K, n0 + RT);
+        }
+        if (t > 0) __syncthreads(); // every wave is done with the previous tile's last slab
+
+        f64x4 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        const int m_base = m0 + wm + (lane & 15), n_base = n0 + wn + (lane >> 4);
+        float b0[2][2][4];
+
+        auto issue = [&](int kb, int st) {
+            unsigned char* S = rd_lds + st * kRdStage;
+            rd_issue<!PT>(a.P, p_si, p_sk, m0, kb, S, wave, lane);
+            rd_issue<QT>(a.Q, q_si, q_sk, n0, kb, S + kRdOpB, wave, lane);
+        };
+        int cur = 0;
+        if (k_lo < k_hi) issue(k_lo, 0);
+        for (int kb = k_lo; kb < k_hi; kb += RK) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            const bool more = kb + RK < k_hi;
+            if (more) issue(kb + RK, cur ^ 1);
+            else {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            b0[i][j][r] = a.B0[(long)(n_base + 16 * j + 4 * r) * a.ldb0 + m_base + 16 * i];
+            }
+            const unsigned char* Ps = rd_lds + cur * kRdStage;
+            const unsigned char* Qs = Ps + kRdOpB;
+            // triangle rule only on a slab that crosses the diagonal (uniform branch)
+            const bool p_diag = a.tri_on == 1 && kb < m0 + RT && m0 < kb + RK;
+            const bool q_diag = a.tri_on == 2 && kb < n0 + RT && n0 < kb + RK;
+#pragma unroll
+            for (int kk = 0; kk < RK; kk += 16) {
+                float pf[2][4], qf[2][4];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) rd_frag<!PT>(Ps, wm + 16 * i + (lane & 15), kk, lane, pf[i]);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) rd_frag<QT>(Qs, wn + 16 * j + (lane & 15), kk, lane, qf[j]);
+                if (p_diag || q_diag) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int k = kb + kk + 4 * (lane >> 4) + s;
+#pragma unroll
+                        for (int i = 0; i < 2; ++i) {
+                            const int x = m0 + wm + 16 * i + (lane & 15); // P(x, k)
+                            if (p_diag) {
+                                float v = (a.tri_lower ? k > x : k < x) ? 0.0f : pf[i][s];
+                                pf[i][s] = (a.tri_unit && k == x) ? 1.0f : v;
+                            }
+                        }
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) {
+                            const int c = n0 + wn + 16 * j + (lane & 15); // Q(k, c)
+                            if (q_diag) {
+                                float v = (a.tri_lower ? c > k : c < k) ? 0.0f : qf[j][s];
+                                qf[j][s] = (a.tri_unit && k == c) ? 1.0f : v;
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    double p[2], q[2];
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) p[i] = (double)pf[i][s];
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) q[j] = (double)qf[j][s];
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) acc[i][j] = icl::mfma_16x16x4_f64(q[j], p[i], acc[i][j]);
+                }
+            }
+            cur ^= 1;
+        }
+        if (k_lo >= k_hi) { // no slab: R = alpha * B0
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        b0[i][j][r] = a.B0[(long)(n_base + 16 * j + 4 * r) * a.ldb0 + m_base + 16 * i];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    a.R[(long)(n_base + 16 * j + 4 * r) * a.ldr + m_base + 16 * i] =
+                        (float)(a.alpha * (double)b0[i][j][r] - acc[i][j][r]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Blocked solve with inverted 64 x 64 diagonal blocks. Step b of the canonical
 // forward substitution, for every group of 64 right-hand sides (lines):
 //   X_b = inv(L_bb) * B_b                          (64 x 64 x 64 FP32 MFMA)
@@ -1101,10 +1304,19 @@ extern "C" hipError_t icl_residual_f64(const icl_residual_args* a, hipStream_t s
                      : a->tri_on == 2 ? tiles_m * ((tiles_n + 1) / 2)
                                       : tiles_m * tiles_n;
     using Kern = void (*)(icl_residual_args);
+    const int which = 2 * (a->p_trans != 0) + (a->q_trans != 0);
+    // The DMA kernel: no edges, and 16-byte aligned operands whose strides are
+    // multiples of 4 floats.
+    auto al16 = [](const void* p, long ld) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0 && ld % 4 == 0; };
+    if (a->M % RT == 0 && a->N % RT == 0 && a->K % RK == 0 && al16(a->P, a->ldp) && al16(a->Q, a->ldq)) {
+        static const Kern dma[4] = {residual_f64_dma_kernel<false, false>, residual_f64_dma_kernel<false, true>,
+                                    residual_f64_dma_kernel<true, false>, residual_f64_dma_kernel<true, true>};
+        hipLaunchKernelGGL(dma[which], dim3((unsigned)tiles), dim3(256), kRdLds, stream, *a);
+        return hipGetLastError();
+    }
     static const Kern kernels[4] = {residual_f64_kernel<false, false>, residual_f64_kernel<false, true>,
                                     residual_f64_kernel<true, false>, residual_f64_kernel<true, true>};
-    hipLaunchKernelGGL(kernels[2 * (a->p_trans != 0) + (a->q_trans != 0)], dim3((unsigned)tiles), dim3(256), 0, stream,
-                       *a);
+    hipLaunchKernelGGL(kernels[which], dim3((unsigned)tiles), dim3(256), 0, stream, *a);
     return hipGetLastError();
 }
 

@@ -110,6 +110,30 @@ def test_strsm_refined_variants(h, v, shape):
     assert rel <= 2.0 ** -23, rel
 
 
+@pytest.mark.parametrize("v", VARIANTS, ids=vid)
+def test_strsm_refined_aligned_variants(h, v):
+    """Refinement on 320 x 192 with leading dimensions that are multiples of 4:
+    the FP64 residual runs on its LDS-DMA kernel (M and N multiples of 64, K of
+    32), the triangle rule applied at the fragment reads of the slabs that cross
+    the diagonal; never-read parts of A are NaN."""
+    side, uplo, trans, diag = v
+    m, n = 320, 192
+    order = m if side == 0 else n
+    A = datagen.triangular(order, lower=bool(uplo), ld=order + 4, seed=44, poison=True)
+    if diag == 1:
+        A[np.arange(order), np.arange(order)] = np.nan
+    B = datagen.matrix(m, n, m + 4, "B", seed=44)
+    B[m:] = -9.0
+    alpha = -0.75
+    X64 = oracle.dtrsm(side, uplo, trans, diag, m, n, alpha, A, B)
+    X = B.copy(order="F")
+    ib.check(ib.iclblasStrsmRefined(h.value, side, uplo, trans, diag, m, n, alpha, A, A.shape[0], X, X.shape[0], 3))
+    assert np.all(X[m:] == -9.0)
+    rel = np.linalg.norm(X[:m] - X64) / np.linalg.norm(X64)
+    assert rel <= 2.0 ** -23, rel
+    assert h.last_refinement_sweeps() >= 1
+
+
 def test_strsm_refined_sweep_count_resets(h):
     """A refined call with no sweeps (or alpha == 0) reports 0 sweeps, whatever
     the handle's previous call ran."""
