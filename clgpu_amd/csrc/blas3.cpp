@@ -587,15 +587,36 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
         const icl_panel_apply_args xa = panel_product(true, t.left, plen, m, n, Wp, B, ldb, Xb, p_lo);
         hip_check(icl_panel_apply(&xa, h->stream), "panel_apply");
         if (pe == t.order) break;
+        // The later unknowns: B_r -= op(A)[r, p] X_p (left) or X_p op(A)[p, r]
+        // (right), a product of depth 256 (plen), on the complex LDS-DMA panel
+        // kernel when the operands allow 16-byte copies, else on a CGEMM.
+        // op(A)(i, j) = A[i si + j sk] (complex elements), conjugated for op C.
         int op = 0;
-        if (t.left) {
+        icl_panel_gemm_args g{};
+        g.nprob = 1;
+        icl_panel_gemm_problem& u = g.prob[0];
+        u.alpha = -1.0f;
+        u.beta = 1.0f;
+        if (t.left) { // D(i, l) = B(r_lo + i, l), P = op(A)[r_lo + i, p_lo + k], Q = X(p_lo + k, l)
             const float* As = t.sub(r_lo, p_lo, &op);
-            cgemm_dev(h, ICL_MATH_FP32, op, 0, rlen, n, plen, minus_one, As, lda, Xb + 2 * (long)p_lo, m, one,
-                      B + 2 * (long)r_lo, ldb);
-        } else {
+            u.rows = rlen, u.lines = n;
+            u.P = As, u.p_is = t.si, u.p_ks = t.sk, u.conj_p = op == 2;
+            u.Q = Xb + 2 * (long)p_lo, u.q_ks = 1, u.q_ls = m;
+            u.D = B + 2 * (long)r_lo, u.d_is = 1, u.d_ls = ldb;
+            if (plen == kPanel && icl_cpanel_gemm_ok(&g)) hip_check(icl_cpanel_gemm(&g, h->stream), "cpanel_gemm");
+            else
+                cgemm_dev(h, ICL_MATH_FP32, op, 0, rlen, n, plen, minus_one, As, lda, Xb + 2 * (long)p_lo, m, one,
+                          B + 2 * (long)r_lo, ldb);
+        } else { // D(i, l) = B(i, r_lo + l), P = X(i, p_lo + k), Q = op(A)[p_lo + k, r_lo + l]
             const float* As = t.sub(p_lo, r_lo, &op);
-            cgemm_dev(h, ICL_MATH_FP32, 0, op, m, rlen, plen, minus_one, Xb + 2 * (long)p_lo * m, m, As, lda, one,
-                      B + 2 * (long)r_lo * ldb, ldb);
+            u.rows = m, u.lines = rlen;
+            u.P = Xb + 2 * (long)p_lo * m, u.p_is = 1, u.p_ks = m;
+            u.Q = As, u.q_ks = t.si, u.q_ls = t.sk, u.conj_q = op == 2;
+            u.D = B + 2 * (long)r_lo * ldb, u.d_is = 1, u.d_ls = ldb;
+            if (plen == kPanel && icl_cpanel_gemm_ok(&g)) hip_check(icl_cpanel_gemm(&g, h->stream), "cpanel_gemm");
+            else
+                cgemm_dev(h, ICL_MATH_FP32, 0, op, m, rlen, plen, minus_one, Xb + 2 * (long)p_lo * m, m, As, lda,
+                          one, B + 2 * (long)r_lo * ldb, ldb);
         }
     }
     hip_check(icl_caxpby_2d(m, n, alpha[0], alpha[1], Xb, m, 0.0f, 0.0f, B, ldb, h->stream), "caxpby");

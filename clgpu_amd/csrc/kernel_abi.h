@@ -177,12 +177,21 @@ typedef struct icl_panel_gemm_problem {
     long d_is, d_ls;
     float alpha, beta;
     int tile0;
+    int conj_p, conj_q; /* complex launches only: use conj(P) / conj(Q) */
 } icl_panel_gemm_problem;
 typedef struct icl_panel_gemm_args {
     int nprob;
     icl_panel_gemm_problem prob[ICL_PANEL_GEMM_MAX];
 } icl_panel_gemm_args;
 hipError_t icl_panel_gemm(icl_panel_gemm_args* a, hipStream_t stream);
+/* The same for interleaved complex matrices (element (i, k) of P at float offset
+ * 2 (i p_is + k p_ks), likewise Q and D), real alpha and beta, conj_p / conj_q
+ * conjugating P / Q. Only the LDS-DMA kernel exists: every operand must allow
+ * 16-byte copies (16-byte aligned, non-unit stride even) and a rows-contiguous
+ * operand have an even row count; otherwise hipErrorInvalidValue and nothing is
+ * launched (icl_cpanel_gemm_ok tells beforehand). */
+hipError_t icl_cpanel_gemm(icl_panel_gemm_args* a, hipStream_t stream);
+int icl_cpanel_gemm_ok(const icl_panel_gemm_args* a);
 
 /* Inverses of the 256 x 256 diagonal panels of the canonical complex system
  * (panel p = unknowns [256 p, 256 p + plen), plen = min(256, nsys - 256 p)),

@@ -283,6 +283,144 @@ __global__ void __launch_bounds__(256, 2) panel_gemm_dma_kernel(icl_panel_gemm_a
     }
 }
 
+// ---- complex LDS-DMA kernel (interleaved re, im) ----
+// The same tiling as panel_gemm_dma_kernel with complex elements: 16-deep
+// chunks of 64 rows (8 KB per operand, as for the real kernel). Rows-contiguous
+// operands are stored S[k][64 rows] (512-byte k rows, one 1 KB copy per two k
+// rows); k-contiguous ones S[row][16 k], 128-byte rows of eight 16-byte chunks
+// of two complex k, chunk c of row r in slot c ^ ((r >> 1) & 7). Step j = 2 g + s
+// of a chunk takes complex k = 2 (2 g + h) + s for lane half h in both operands,
+// so one 16-byte read of a k-contiguous operand feeds two steps; a step is four
+// real MFMAs: re += Pr Qr - Pi Qi, im += Pr Qi + Pi Qr (conjugation negates Pi or Qi).
+constexpr int kCdKC = 16;
+constexpr int kCdChunks = 256 / kCdKC;
+
+// rows % 2 == 0 for ROWC (each copy moves two complex rows), checked by the host.
+template <bool ROWC>
+__device__ __forceinline__ void cd_issue(const float* X, long rs, long ks, int row0, int k0, int rows,
+                                         unsigned char* S, int wave, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < kPdBlocksPerWave; ++i) {
+        const int blk = wave + 4 * i;
+        const float* g;
+        if constexpr (ROWC) {
+            const int kr = 2 * blk + (lane >> 5);                  // k row; lane covers rows 2 (lane % 32), +1
+            const int r = min(row0 + 2 * (lane & 31), rows - 2);
+            g = X + 2 * ((long)(k0 + kr) * ks + r);
+        } else {
+            const int r = 8 * blk + (lane >> 3);
+            const int c = (lane & 7) ^ pd_swizzle(r);
+            g = X + 2 * ((long)min(row0 + r, rows - 1) * rs + k0 + 2 * c);
+        }
+        __builtin_amdgcn_global_load_lds((pg_global_void*)g, (pg_lds_void*)(S + blk * 1024), 16, 0, 0);
+    }
+}
+
+// re[s], im[s] = X(rb + lane % 32, k), k = 2 (2 g + h) + s, s = 0, 1.
+template <bool ROWC>
+__device__ __forceinline__ void cd_frag(const unsigned char* S, int rb, int g, int lane, float (&re)[2],
+                                        float (&im)[2])
+{
+    const int h = lane >> 5, r = rb + (lane & 31), q = 2 * g + h;
+    if constexpr (ROWC) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const f32x2 v = *reinterpret_cast<const f32x2*>(S + (2 * q + s) * 512 + r * 8);
+            re[s] = v[0];
+            im[s] = v[1];
+        }
+    } else {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(S + r * 128 + ((q ^ pd_swizzle(r)) << 4));
+        re[0] = v[0], im[0] = v[1], re[1] = v[2], im[1] = v[3];
+    }
+}
+
+template <bool P_I, bool Q_K, bool D_I, bool EDGE>
+__global__ void __launch_bounds__(256, 2) cpanel_gemm_dma_kernel(icl_panel_gemm_args a)
+{
+    extern __shared__ __attribute__((aligned(1024))) unsigned char cd_lds[];
+    const int b = blockIdx.x;
+    int pi = 0;
+    for (int i = 1; i < a.nprob; ++i)
+        if (b >= a.prob[i].tile0) pi = i;
+    const icl_panel_gemm_problem& pr = a.prob[pi];
+    const int rows = pr.rows, lines = pr.lines;
+    const int tiles_i = (rows + kPgT - 1) / kPgT;
+    const int t = b - pr.tile0;
+    const int i0 = (t % tiles_i) * kPgT, l0 = (t / tiles_i) * kPgT;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wi = 32 * (wave & 1), wl = 32 * (wave >> 1);
+    const float sp = pr.conj_p ? -1.0f : 1.0f, sq = pr.conj_q ? -1.0f : 1.0f;
+
+    auto issue = [&](int c, int s) {
+        unsigned char* st = cd_lds + s * kPdStage;
+        cd_issue<P_I>(pr.P, pr.p_is, pr.p_ks, i0, c * kCdKC, EDGE ? rows : 1 << 30, st, wave, lane);
+        cd_issue<!Q_K>(pr.Q, pr.q_ls, pr.q_ks, l0, c * kCdKC, EDGE ? lines : 1 << 30, st + kPdOpB, wave, lane);
+    };
+    auto d_index = [&](int e, int& i, int& l) {
+        const int u = icl::mfma32_row(lane, e), v = lane & 31;
+        i = i0 + wi + (D_I ? v : u);
+        l = l0 + wl + (D_I ? u : v);
+    };
+    const float alpha = pr.alpha, beta = pr.beta;
+    f32x2 dold[16];
+
+    f32x16 ar = f32x16{}, ai = f32x16{};
+    issue(0, 0);
+    for (int c = 0; c < kCdChunks; ++c) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (c + 1 < kCdChunks) issue(c + 1, (c + 1) & 1);
+        if (c + 1 == kCdChunks && beta != 0.0f) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                int i, l;
+                d_index(e, i, l);
+                dold[e] = (!EDGE || (i < rows && l < lines))
+                              ? *reinterpret_cast<const f32x2*>(pr.D + 2 * ((long)i * pr.d_is + (long)l * pr.d_ls))
+                              : f32x2{0.0f, 0.0f};
+            }
+        }
+        const unsigned char* Ps = cd_lds + (c & 1) * kPdStage;
+        const unsigned char* Qs = Ps + kPdOpB;
+#pragma unroll
+        for (int g = 0; g < kCdKC / 4; ++g) {
+            float pr_[2], pi_[2], qr[2], qi[2];
+            cd_frag<P_I>(Ps, wi, g, lane, pr_, pi_);
+            cd_frag<!Q_K>(Qs, wl, g, lane, qr, qi);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const float p_im = sp * pi_[s], q_im = sq * qi[s];
+                if constexpr (D_I) { // b operand: P (lanes along i)
+                    ar = icl::mfma_32x32x2_f32(qr[s], pr_[s], ar);
+                    ar = icl::mfma_32x32x2_f32(-q_im, p_im, ar);
+                    ai = icl::mfma_32x32x2_f32(qr[s], p_im, ai);
+                    ai = icl::mfma_32x32x2_f32(q_im, pr_[s], ai);
+                } else {
+                    ar = icl::mfma_32x32x2_f32(pr_[s], qr[s], ar);
+                    ar = icl::mfma_32x32x2_f32(-p_im, q_im, ar);
+                    ai = icl::mfma_32x32x2_f32(pr_[s], q_im, ai);
+                    ai = icl::mfma_32x32x2_f32(p_im, qr[s], ai);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        int i, l;
+        d_index(e, i, l);
+        if (EDGE && (i >= rows || l >= lines)) continue;
+        float xr = alpha * ar[e], xi = alpha * ai[e];
+        if (beta != 0.0f) {
+            xr = fmaf(beta, dold[e][0], xr);
+            xi = fmaf(beta, dold[e][1], xi);
+        }
+        *reinterpret_cast<f32x2*>(pr.D + 2 * ((long)i * pr.d_is + (long)l * pr.d_ls)) = f32x2{xr, xi};
+    }
+}
+
 using PgKern = void (*)(icl_panel_gemm_args);
 
 template <bool P_I, bool Q_K, bool D_I>
@@ -333,5 +471,68 @@ extern "C" hipError_t icl_panel_gemm(icl_panel_gemm_args* a, hipStream_t stream)
     else if (p_i && !q_k && d_i) k = pick<true, false, true>(edge, vec, dma);   // N_p, right op T
     else return hipErrorInvalidValue;
     hipLaunchKernelGGL(k, dim3((unsigned)tiles), dim3(256), dma ? kPdLds : 0, stream, *a);
+    return hipGetLastError();
+}
+
+namespace {
+
+// Layout of a complex launch: the contiguous dimensions, alike in all problems,
+// and whether every operand allows the 16-byte copies.
+bool cpanel_layout(const icl_panel_gemm_args* a, bool* p_i, bool* q_k, bool* d_i)
+{
+    if (a->nprob <= 0 || a->nprob > ICL_PANEL_GEMM_MAX) return false;
+    const icl_panel_gemm_problem& f = a->prob[0];
+    *p_i = f.p_is == 1, *q_k = f.q_ks == 1, *d_i = f.d_is == 1;
+    for (int i = 0; i < a->nprob; ++i) {
+        const icl_panel_gemm_problem& p = a->prob[i];
+        if (p.rows < 0 || p.lines < 0) return false;
+        if ((p.p_is == 1) != *p_i || (!*p_i && p.p_ks != 1) || (p.q_ks == 1) != *q_k || (!*q_k && p.q_ls != 1) ||
+            (p.d_is == 1) != *d_i || (!*d_i && p.d_ls != 1))
+            return false;
+        // 16-byte copies: aligned bases, even non-unit strides (two complex per 16 bytes)
+        auto ok = [](const void* x, long stride) { return (reinterpret_cast<uintptr_t>(x) & 15u) == 0 && stride % 2 == 0; };
+        if (!ok(p.P, *p_i ? p.p_ks : p.p_is) || !ok(p.Q, *q_k ? p.q_ls : p.q_ks)) return false;
+        if ((*p_i && p.rows % 2 != 0) || (!*q_k && p.lines % 2 != 0)) return false;
+    }
+    const bool supported = (*p_i && *q_k && *d_i) || (!*p_i && *q_k && *d_i) || (*p_i && !*q_k && *d_i) ||
+                           (!*p_i && !*q_k && !*d_i);
+    return supported;
+}
+
+template <bool P_I, bool Q_K, bool D_I>
+PgKern cpick(bool edge)
+{
+    return edge ? cpanel_gemm_dma_kernel<P_I, Q_K, D_I, true> : cpanel_gemm_dma_kernel<P_I, Q_K, D_I, false>;
+}
+
+} // namespace
+
+extern "C" int icl_cpanel_gemm_ok(const icl_panel_gemm_args* a)
+{
+    bool p_i, q_k, d_i;
+    return cpanel_layout(a, &p_i, &q_k, &d_i) ? 1 : 0;
+}
+
+extern "C" hipError_t icl_cpanel_gemm(icl_panel_gemm_args* a, hipStream_t stream)
+{
+    if (a->nprob <= 0) return hipSuccess;
+    bool p_i, q_k, d_i;
+    if (!cpanel_layout(a, &p_i, &q_k, &d_i)) return hipErrorInvalidValue;
+    long tiles = 0;
+    bool edge = false;
+    for (int i = 0; i < a->nprob; ++i) {
+        icl_panel_gemm_problem& p = a->prob[i];
+        p.tile0 = (int)tiles;
+        tiles += (long)((p.rows + kPgT - 1) / kPgT) * ((p.lines + kPgT - 1) / kPgT);
+        edge = edge || p.rows % kPgT != 0 || p.lines % kPgT != 0;
+    }
+    if (tiles == 0) return hipSuccess;
+    if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
+    PgKern k = nullptr;
+    if (p_i && q_k && d_i) k = cpick<true, true, true>(edge);           // left op N, right op N
+    else if (!p_i && q_k && d_i) k = cpick<false, true, true>(edge);    // left op T / C
+    else if (p_i && !q_k && d_i) k = cpick<true, false, true>(edge);    // right op T / C
+    else k = cpick<false, false, false>(edge);
+    hipLaunchKernelGGL(k, dim3((unsigned)tiles), dim3(256), kPdLds, stream, *a);
     return hipGetLastError();
 }
