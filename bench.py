@@ -642,8 +642,11 @@ def sgemm_line(a, h, ib, stream, dev):
             step()
         torch.cuda.synchronize()
         # The library records HIP events around the main GEMM kernel of every call
-        # (the handle's stream, the one the kernel runs on; the split path's
-        # operand pack stays outside), over the same timed region as the call.
+        # (the handle's stream, the one the kernel runs on), over the same timed
+        # region as the call. At 8192^2 the split path packs the first halves of
+        # its operands outside the pair, then runs its product as three launches
+        # (A0 x B0 beside the pack of the second halves on a second stream, then
+        # A0 x B1 and A1 x B), all inside the pair.
         h.set_kernel_timing(True)
         start = torch.cuda.Event(enable_timing=True)
         stop = torch.cuda.Event(enable_timing=True)

@@ -15,6 +15,7 @@
 namespace icl {
 
 thread_local KernelTimer* t_active_timer = nullptr;
+thread_local const icl_side_lane* t_side_lane = nullptr;
 
 // ---------------------------------------------------------------------------
 // Registry
@@ -150,8 +151,16 @@ static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, co
             ws = h->scratch[iclblasContext::kSlotGemmWorkspace].get(bytes, h->stream);
         }
     }
-    // A variant with a prepass marks its main kernel itself (icl_timer_mark).
+    // A variant with a prepass marks its main kernel itself (icl_timer_mark),
+    // and may run half of the prepass on the handle's side stream: offered for
+    // products of at least 1024 tiles of that variant, never to a call that must
+    // not allocate (the stream and its events are created on first use).
     if (!v->workspace) hip_check(icl_timer_mark(h->stream), "icl_timer_mark");
+    const long tiles = (long)((g.M + v->bm - 1) / v->bm) * ((g.N + v->bn - 1) / v->bn);
+    struct SideGuard {
+        explicit SideGuard(const icl_side_lane* l) { t_side_lane = l; }
+        ~SideGuard() { t_side_lane = nullptr; }
+    } side_guard(v->workspace && ws && !h->ext_ws && tiles >= 1024 && !level3_problem(g) ? h->side.ensure() : nullptr);
     hip_check(v->launch(&g, opa, opb, edge ? 1 : 0, ws, h->stream), v->name);
     if (!v->workspace) hip_check(icl_timer_mark(h->stream), "icl_timer_mark");
     if (record) h->last_impl = v->name;
@@ -1076,6 +1085,8 @@ void strsm_backward_dev(iclblasContext* h, int side, int uplo, int trans, int di
 }
 
 } // namespace icl
+
+extern "C" const icl_side_lane* icl_gemm_side_lane(void) { return icl::t_side_lane; }
 
 extern "C" hipError_t icl_timer_mark(hipStream_t stream)
 {

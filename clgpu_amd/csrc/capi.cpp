@@ -98,6 +98,7 @@ ICLBLAS_API iclblasStatus_t iclblasDestroy(iclblasHandle_t handle)
     (void)hipSetDevice(handle->device);
     (void)hipStreamSynchronize(handle->stream);
     handle->copy.release();
+    handle->side.release();
     for (auto& s : handle->scratch) s.release();
     if (handle->trsm_error) (void)hipFree(handle->trsm_error);
     if (handle->own_stream) (void)hipStreamDestroy(handle->stream);

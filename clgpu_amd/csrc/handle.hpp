@@ -16,6 +16,7 @@
 
 #include "iclBLAS.h"
 #include "iclBLAS_ext.h"
+#include "kernel_abi.h"
 
 namespace icl {
 
@@ -151,6 +152,37 @@ struct CopyLane {
     }
 };
 
+// The side stream of a handle's large split-BF16 GEMMs (icl_side_lane): the
+// second half of the operand prepass runs there while the first products run
+// on the handle's stream. Non-blocking, default priority; created on first use.
+struct SideLane {
+    icl_side_lane lane{};
+    SideLane() = default;
+    SideLane(const SideLane&) = delete;
+    SideLane& operator=(const SideLane&) = delete;
+    ~SideLane() { release(); }
+
+    const icl_side_lane* ensure()
+    {
+        if (lane.stream) return &lane;
+        hip_check(hipEventCreateWithFlags(&lane.start, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&lane.done, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipStreamCreateWithFlags(&lane.stream, hipStreamNonBlocking), "hipStreamCreate");
+        return &lane;
+    }
+
+    void release()
+    {
+        if (lane.stream) {
+            (void)hipStreamSynchronize(lane.stream);
+            (void)hipStreamDestroy(lane.stream);
+        }
+        if (lane.start) (void)hipEventDestroy(lane.start);
+        if (lane.done) (void)hipEventDestroy(lane.done);
+        lane = icl_side_lane{};
+    }
+};
+
 // Opt-in timing of the main kernel of each GEMM launch (iclblasSetKernelTiming):
 // a begin/end HIP event pair recorded on the launch stream around that kernel
 // alone. Events are created on first use and reused; at most kMaxEvents are
@@ -193,6 +225,9 @@ struct KernelTimer {
 // The timer of the handle whose call the current thread is running (set for the
 // duration of a call when that handle's timing is on), read by icl_timer_mark.
 extern thread_local KernelTimer* t_active_timer;
+// The side lane offered to the GEMM launch the current thread is running (set by
+// run_gemm for the duration of one launch), read by icl_gemm_side_lane.
+extern thread_local const icl_side_lane* t_side_lane;
 
 } // namespace icl
 
@@ -220,6 +255,7 @@ struct iclblasContext {
     icl::DeviceScratch scratch[kScratchSlots];
     icl::PinnedStager stager;
     icl::CopyLane copy;
+    icl::SideLane side;
     int* trsm_error = nullptr; // device word set by an aborted dataflow STRSM (allocated on first use)
     bool trsm_error_pending = false; // a dataflow STRSM was queued since the word was last read
     icl::KernelTimer timer;

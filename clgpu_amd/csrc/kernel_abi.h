@@ -86,6 +86,15 @@ typedef struct icl_gemm_variant {
  * prepass stays outside the timed pair. */
 hipError_t icl_timer_mark(hipStream_t stream);
 
+/* A second stream of the handle whose call is running, with two events, on
+ * which a GEMM may run its operand prepass beside its first products (the
+ * packed split-BF16 GEMMs); NULL when the call has none. */
+typedef struct icl_side_lane {
+    hipStream_t stream;
+    hipEvent_t start, done;
+} icl_side_lane;
+const icl_side_lane* icl_gemm_side_lane(void);
+
 /* Registered GEMM implementations. Kernel translation units register their
  * tables at library load (icl_register_gemm_variants); the host layer reads
  * them back with icl_gemm_variant_count / icl_gemm_variant_at, ordered by name. */

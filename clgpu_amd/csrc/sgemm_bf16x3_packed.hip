@@ -30,8 +30,9 @@ constexpr int kPackK = 32; // k per slab
 struct PackSrc {
     const float* X;
     long ld;
-    int R, K, Rpad, kcontig;
+    int R, K, Rpad, kcontig; // Rpad: rows of the packed layout (its slab stride)
     __bf16* P;
+    int blocks;              // 64-row blocks packed from X / P (rows R.. of them are zero padding)
 };
 
 __global__ void __launch_bounds__(256) split_pack_kernel(PackSrc a, PackSrc b, int ablocks, int slabs)
@@ -86,9 +87,10 @@ __global__ void __launch_bounds__(256) split_pack_kernel(PackSrc a, PackSrc b, i
 
 struct PackedArgs {
     icl_gemm_args g; // M, N, C, ldc, alpha, beta, mask (A/B/K fields unused here)
-    const __bf16* Ap;
+    const __bf16* Ap; // row 0 of this launch in the packed op(A)
     const __bf16* Bp;
-    int Mpad, Npad, slabs;
+    int Mpad, Npad, slabs;  // tile grid of this launch (M, N padded), slabs of K
+    int a_rows, b_rows;     // rows of the packed layouts (their slab strides)
 };
 
 // ---------------------------------------------------------------------------
@@ -144,8 +146,8 @@ __global__ void __launch_bounds__(WVM* WVN * 64) sgemm_bf16x3_dma_kernel(PackedA
         ks1 = ke > kb ? (ke + kPackK - 1) / kPackK : ks0;
     }
 
-    const size_t a_slab = (size_t)p.Mpad * (2 * kPackK);
-    const size_t b_slab = (size_t)p.Npad * (2 * kPackK);
+    const size_t a_slab = (size_t)p.a_rows * (2 * kPackK);
+    const size_t b_slab = (size_t)p.b_rows * (2 * kPackK);
     const __bf16* a_src = p.Ap + (size_t)m0 * (2 * kPackK);
     const __bf16* b_src = p.Bp + (size_t)n0 * (2 * kPackK);
     // lane -> (row within an 8-row DMA block, logical chunk)
@@ -239,10 +241,21 @@ size_t packed_workspace(const icl_gemm_args* g)
 
 hipError_t pack(const PackSrc& a, const PackSrc& b, int slabs, hipStream_t s)
 {
-    const int ablocks = a.Rpad / 64;
-    const dim3 grid((unsigned)(ablocks + b.Rpad / 64), (unsigned)((slabs + 1) / 2));
-    hipLaunchKernelGGL(split_pack_kernel, grid, dim3(256), 0, s, a, b, ablocks, slabs);
+    const dim3 grid((unsigned)(a.blocks + b.blocks), (unsigned)((slabs + 1) / 2));
+    hipLaunchKernelGGL(split_pack_kernel, grid, dim3(256), 0, s, a, b, a.blocks, slabs);
     return hipGetLastError();
+}
+
+// The rows [r0, r0 + rows) of a source and its packed layout as a PackSrc of
+// their own (r0 a multiple of 64; rows up to rpad_rows are padding).
+PackSrc pack_rows(const PackSrc& whole, int r0, int rows, int rpad_rows)
+{
+    PackSrc s = whole;
+    s.X = whole.X + (whole.kcontig ? (long)r0 * whole.ld : (long)r0);
+    s.R = rows;
+    s.P = whole.P + (size_t)r0 * (2 * kPackK);
+    s.blocks = rpad_rows / 64;
+    return s;
 }
 
 template <int BM, int BN, int WVM, int WVN>
@@ -254,20 +267,64 @@ hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, voi
     p.Mpad = round_up(g->M, BM);
     p.Npad = round_up(g->N, BN);
     p.slabs = round_up(g->K, kPackK) / kPackK;
+    p.a_rows = p.Mpad;
+    p.b_rows = p.Npad;
     __bf16* Ap = static_cast<__bf16*>(ws);
     __bf16* Bp = Ap + (size_t)p.Mpad * p.slabs * (2 * kPackK);
     p.Ap = Ap;
     p.Bp = Bp;
     // op(A)(m, k): N -> A[m + k*lda] (rows contiguous), T/C -> A[k + m*lda] (k contiguous)
-    const PackSrc pa{g->A, g->lda, g->M, g->K, p.Mpad, opa != 0, Ap};
+    const PackSrc pa{g->A, g->lda, g->M, g->K, p.Mpad, opa != 0, Ap, p.Mpad / 64};
     // op(B)(k, n) as rows n: N -> B[k + n*ldb] (k contiguous), T/C -> B[n + k*ldb]
-    const PackSrc pb{g->B, g->ldb, g->N, g->K, p.Npad, opb == 0, Bp};
-    hipError_t e = pack(pa, pb, p.slabs, s);
+    const PackSrc pb{g->B, g->ldb, g->N, g->K, p.Npad, opb == 0, Bp, p.Npad / 64};
+    using C = DmaCfg<BM, BN, WVM, WVN>;
+    (void)edge;
+    hipError_t e;
+
+    // Large plain products with a side stream: the operands in halves (rows of
+    // op(A), rows of op(B)^T). The handle's stream packs A0 and B0 and multiplies
+    // A0 x B0 while the side stream packs A1 and B1; then A0 x B1 and A1 x B
+    // follow, once the side stream is done. Only the first halves' pack is
+    // exposed, instead of the whole prepass. Each launch computes its elements
+    // exactly as the single launch would (same slabs, same order).
+    const icl_side_lane* side = icl_gemm_side_lane();
+    const int m0 = round_up(g->M / 2, BM), n0 = round_up(g->N / 2, BN);
+    if (side && !uses_level3(g) && g->M % BM == 0 && g->N % BN == 0 && m0 < g->M && n0 < g->N) {
+        // The side stream starts after the first halves' pack, so the two packs do
+        // not share the memory bandwidth: the second runs beside A0 x B0, which
+        // leaves registers and LDS for its workgroups on every CU.
+        if ((e = pack(pack_rows(pa, 0, m0, m0), pack_rows(pb, 0, n0, n0), p.slabs, s)) != hipSuccess) return e;
+        if ((e = hipEventRecord(side->start, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(side->stream, side->start, 0)) != hipSuccess) return e;
+        if ((e = pack(pack_rows(pa, m0, g->M - m0, p.Mpad - m0), pack_rows(pb, n0, g->N - n0, p.Npad - n0), p.slabs,
+                      side->stream)) != hipSuccess)
+            return e;
+        if ((e = hipEventRecord(side->done, side->stream)) != hipSuccess) return e;
+        if ((e = icl_timer_mark(s)) != hipSuccess) return e;
+        // (row offset, rows, column offset, columns) of C per launch
+        const int part[3][4] = {{0, m0, 0, n0}, {0, m0, n0, g->N - n0}, {m0, g->M - m0, 0, g->N}};
+        for (int q = 0; q < 3; ++q) {
+            if (q == 1 && (e = hipStreamWaitEvent(s, side->done, 0)) != hipSuccess) return e;
+            icl_gemm_args gq = *g;
+            gq.M = part[q][1];
+            gq.N = part[q][3];
+            gq.C = g->C + (long)part[q][2] * g->ldc + part[q][0];
+            PackedArgs pq = p;
+            pq.g = gq;
+            pq.Mpad = gq.M;
+            pq.Npad = gq.N;
+            pq.Ap = Ap + (size_t)part[q][0] * (2 * kPackK);
+            pq.Bp = Bp + (size_t)part[q][2] * (2 * kPackK);
+            e = launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false>, &gq, s, pq);
+            if (e != hipSuccess) return e;
+        }
+        return icl_timer_mark(s);
+    }
+
+    e = pack(pa, pb, p.slabs, s);
     if (e != hipSuccess) return e;
     const bool even = g->M % BM == 0 && g->N % BN == 0;
-    (void)edge;
     // launch_gemm sizes the grid from (M, N); the packed grid is the same.
-    using C = DmaCfg<BM, BN, WVM, WVN>;
     if ((e = icl_timer_mark(s)) != hipSuccess) return e;
     if (uses_level3(g))
         e = even ? launch_gemm<BM, BN, C::NT, C::LDS_BYTES>(sgemm_bf16x3_dma_kernel<BM, BN, WVM, WVN, false, true>, g,

@@ -112,7 +112,10 @@ ICLBLAS_API iclblasStatus_t iclblasStrsmBackward(iclblasHandle_t handle, iclblas
 
 /* Kernel timing (a measurement aid). While enabled, every GEMM the handle runs
  * records a pair of HIP events on the handle's stream around its main kernel
- * alone (the operand pack of the split-BF16 kernels runs outside the pair).
+ * alone. The operand pack of the split-BF16 kernels runs outside the pair,
+ * except, for products of at least 1024 tiles, the half of it that runs on the
+ * handle's second stream beside the first partial product (the pair spans the
+ * three partial products and the wait for that half).
  * iclblasSetKernelTiming waits for the handle's stream and starts a new sum.
  * iclblasGetKernelTiming waits for the stream, returns the summed duration of
  * the recorded pairs and their number, and starts a new sum; at most 4096

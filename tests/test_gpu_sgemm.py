@@ -274,6 +274,42 @@ def test_sgemm_config2_nonsquare_padded(dev_handle, mode):
     h.set_math_mode(ib.MATH_FP32)
 
 
+@pytest.mark.parametrize("ta,tb,m", [(0, 0, 8192), (1, 1, 8192), (0, 1, 8448), (1, 0, 8448)])
+def test_sgemm_split_pack_overlap_bit_identical(dev_handle, ta, tb, m):
+    """A large split-BF16 product (>= 1024 tiles) packs half of its operands on
+    the handle's side stream and runs as four quadrant launches; the
+    allocation-free entry point runs the same kernel in one launch after a
+    single prepass. Both give the same bits (same per-element summation order),
+    including an M whose halves are not equal (8448 = 4352 + 4096) and a K that
+    is not a multiple of the 32-deep slab."""
+    import torch
+    n, k, ld = 8192, 300, 8512
+    rows_a, cols_a = (m, k) if ta == 0 else (k, m)
+    rows_b, cols_b = (k, n) if tb == 0 else (n, k)
+    A = datagen.matrix(ld, cols_a, name="A", seed=21)
+    B = datagen.matrix(ld, cols_b, name="B", seed=21)
+    C = datagen.matrix(ld, n, name="C", seed=21)
+    dA, dB = _to_dev(A), _to_dev(B)
+    outs = []
+    for no_alloc in (False, True):
+        h = dev_handle
+        h.set_math_mode(ib.MATH_SPLIT_BF16X3)
+        dC = _to_dev(C)
+        fn = ib.iclblasSgemmNoAlloc if no_alloc else ib.iclblasSgemm
+        ib.check(fn(h.value, ta, tb, m, n, k, 0.5, dA.data_ptr(), ld, dB.data_ptr(), ld, -1.0, dC.data_ptr(), ld))
+        torch.cuda.synchronize()
+        assert h.last_implementation().startswith("sgemm_bf16x3d")
+        outs.append(dC.cpu().numpy())
+        h.set_math_mode(ib.MATH_FP32)
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0][:, m:], np.ascontiguousarray(C.T)[:, m:])  # rows m.. of C untouched
+    rng = np.random.default_rng(5)
+    rows, cols = rng.integers(0, m, 256), rng.integers(0, n, 256)
+    got = outs[0].T[rows, cols].astype(np.float64)
+    c64, ab = oracle.dgemm_sample(ta, tb, k, 0.5, A, B, rows, cols, beta=-1.0, C=C)
+    assert bound_ratio(np.abs(got - c64), k, 0.5 * ab, U32 * np.abs(C[rows, cols].astype(np.float64))) <= 4.0
+
+
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tb", [0, 1], ids=["BN", "BT"])
 @pytest.mark.parametrize("beta", [0.0, -0.5], ids=["beta0", "beta"])
