@@ -36,7 +36,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
@@ -136,10 +136,21 @@ def cpu_baseline(A: np.ndarray, B: np.ndarray, n: int, target_s: float) -> dict:
                                            if use_ref else "the restated cpu_gemm on one thread")}}
 
 
-def _time(stream, fn, steps: int, warmup: int) -> float:
+def _time(stream, fn, steps: int, warmup: int, settle_s: float = 0.25, warm_fn=None) -> float:
+    """Mean ms per call over `steps` calls of fn, after `warmup` calls of warm_fn
+    (default fn) and then as many more as keep the GPU busy for `settle_s`: an
+    idle GPU's clock ramps up over the first tenths of a second of load, so a row
+    timed right after host-side set-up would otherwise run its first calls at a
+    lower clock."""
     import torch
-    for _ in range(warmup):
-        fn()
+    warm = warm_fn or fn
+    t0 = time.perf_counter()
+    n = 0
+    while n < warmup or time.perf_counter() - t0 < settle_s:
+        warm()
+        n += 1
+        if n % 4 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -149,6 +160,23 @@ def _time(stream, fn, steps: int, warmup: int) -> float:
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def _solve_timer(stream, solve, rhs, steps: int) -> float:
+    """Mean ms of an in-place solve, solve(X), over `steps` calls, each on its own
+    copy of the right-hand side rhs made before the timed region (so every timed
+    call solves the same data and no copy is timed); the warm-up solves a scratch
+    copy restored before each call."""
+    pool = [rhs.clone() for _ in range(steps)]
+    it = iter(pool)
+    scratch = rhs.clone()
+
+    def warm():
+        scratch.copy_(rhs)
+        solve(scratch)
+    ms = _time(stream, lambda: solve(next(it)), steps, 1, warm_fn=warm)
+    del pool, scratch
+    return ms
 
 
 def sgemm_rows(h, ib, stream, steps: int) -> dict:
@@ -230,13 +258,13 @@ def ctrsm_rows(h, ib, stream, steps: int) -> dict:
         ib.check(ib.iclblasCtrsm(h.value, 0, 1, trans, 0, m, nr, 1.0, tT.data_ptr(), m, X.data_ptr(), m))
         torch.cuda.synchronize()
         fe = float((X.to(torch.complex128).t() - X64).norm() / X64.norm())
-        Xw = tB.clone()
-        ms = _time(stream, lambda: ib.check(ib.iclblasCtrsm(h.value, 0, 1, trans, 0, m, nr, 1.0, tT.data_ptr(), m,
-                                                            Xw.data_ptr(), m)), max(2, steps // 4), 1)
+        ms = _solve_timer(stream, lambda Xw: ib.check(ib.iclblasCtrsm(h.value, 0, 1, trans, 0, m, nr, 1.0,
+                                                                      tT.data_ptr(), m, Xw.data_ptr(), m)),
+                          tB, max(2, steps // 4))
         tf = 4.0 * m * m * nr / (ms * 1e-3) / 1e12
         out[label] = {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
                       "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA, "forward_rel_err_fro": fe}
-        del X, Xw, X64, opL
+        del X, X64, opL
     h.set_implementation(None)
     del tT, tB, L64
     torch.cuda.empty_cache()
@@ -557,8 +585,8 @@ def strsm_rows(h, ib, stream, steps: int) -> dict:
     """STRSM left/lower/N/non-unit 4096 x 1024, plain and with at most 3
     refinement sweeps (they stop once a correction no longer changes X at FP32
     resolution; `sweeps_run` reports how many ran); forward error against an
-    FP64 solve computed with torch on the GPU. Every timed call first restores the
-    same right-hand side (a 4 MiB device copy, inside the timed region)."""
+    FP64 solve computed with torch on the GPU. Every timed call solves its own copy
+    of the same right-hand side, made before the timed region."""
     import torch
 
     from clgpu_amd import datagen
@@ -580,18 +608,13 @@ def strsm_rows(h, ib, stream, steps: int) -> dict:
         torch.cuda.synchronize()
         fe = float((X.double().t() - X64).norm() / X64.norm())
         sweeps_run = h.last_refinement_sweeps()
-        Xw = tB.clone()
-
-        def call():
-            Xw.copy_(tB)
-            ib.check(ib.iclblasStrsmRefined(h.value, 0, 1, 0, 0, m, nr, 1.0, tT.data_ptr(), m, Xw.data_ptr(), m,
-                                            sweeps))
-        ms = _time(stream, call, max(2, steps // 4), 1)
+        ms = _solve_timer(stream, lambda Xw: ib.check(ib.iclblasStrsmRefined(
+            h.value, 0, 1, 0, 0, m, nr, 1.0, tT.data_ptr(), m, Xw.data_ptr(), m, sweeps)), tB, max(2, steps // 4))
         tf = flops_t / (ms * 1e-3) / 1e12
         ts[label] = {"implementation": h.last_implementation(), "ms": ms, "tflops": tf,
                      "frac_of_fp32_mfma_peak": tf / PEAK_FP32_MFMA, "forward_rel_err_fro": fe,
                      "sweeps_run": sweeps_run}
-    del tT, tB, L64, X64, X, Xw
+    del tT, tB, L64, X64, X
     torch.cuda.empty_cache()
     return ts
 

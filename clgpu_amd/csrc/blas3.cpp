@@ -54,15 +54,20 @@ bool gemm_needs_edge(const icl_gemm_variant& v, const GemmProblem& p)
 }
 
 // Score = 1 / estimated run time (seconds). Each variant carries its measured
-// full-GPU rate R (TFLOP/s at 8192^3) and occupancy occ (workgroups per CU).
-// With 256 CUs, 256*occ tiles run concurrently, each tile taking
-// t_tile = 2*bm*bn*K*flops_per_mac_scale / (R / (256*occ)) when the CUs are
-// full; the grid runs in ceil(tiles / (256*occ)) rounds, so ragged problems pay
-// for padded tiles. A last round with fewer than 256*occ tiles shares the CUs
-// among fewer workgroups: it is taken to last t_tile * max(tiles_last, 256) /
-// (256*occ), so grids of fewer tiles than CUs favour smaller tiles. Variants
-// with a pack prepass add its HBM time ((M + N) * K * (4 read + 4 written)
-// bytes at 5 TB/s). The bounds-checked instantiation is taken to run 10% slower.
+// full-GPU rate R (TFLOP/s at 8192^3 of its main kernel) and occupancy occ
+// (workgroups per CU). With 256 CUs, 256*occ tiles run concurrently, each tile
+// taking t_tile = 2*bm*bn*K*flops_per_mac_scale / (R / (256*occ)) when the CUs
+// are full; the grid runs in ceil(tiles / (256*occ)) rounds, so ragged problems
+// pay for padded tiles. A last round with fewer than 256*occ tiles shares the
+// CUs among fewer workgroups: it is taken to last t_tile * max(tiles_last, 256)
+// / (256*occ), so grids of fewer tiles than CUs favour smaller tiles. A grid
+// that fills only a fraction f of the CUs from the start (a single round) runs
+// its tiles faster than a full chip does (a higher clock at the lower power,
+// less contention for L2 and HBM): (0.7 + 0.3 f) of that time, fitted to SSYRK
+// and CHERK triangles of 72 to 156 tiles. Variants with a pack prepass add its
+// HBM time ((M + N) * K elements at 4 read + 4 written bytes, twice that
+// complex, at 5 TB/s). The bounds-checked instantiation is taken to run 10%
+// slower.
 // For K <= 512 a tile's fixed cost (pipeline fill, C read and write) is no
 // longer small against its K loop: it is counted as kTileOverheadK * (elements
 // of C per wave) extra steps of K, normalised so that the rate at K = 8192 is
@@ -88,8 +93,9 @@ float gemm_score(const icl_gemm_variant& v, const GemmProblem& p)
     const double rounds = std::ceil(tiles / conc);
     const double last = tiles - (rounds - 1.0) * conc;
     double t = (rounds - 1.0) * t_tile + t_tile * std::max(last, kCUs) / conc;
+    if (rounds == 1.0) t *= 0.7 + 0.3 * std::min(tiles, kCUs) / kCUs;
     if (gemm_needs_edge(v, p)) t *= 1.10;
-    if (v.workspace) t += (double(g.M) + double(g.N)) * double(g.K) * 8.0 * macs / 5e12;
+    if (v.workspace) t += (double(g.M) + double(g.N)) * double(g.K) * (p.kind == ICL_KIND_CGEMM ? 16.0 : 8.0) / 5e12;
     return (float)(1.0 / t);
 }
 

@@ -288,7 +288,7 @@ hipError_t cpacked_launch(const icl_gemm_args* g, int opa, int opb, int edge, vo
      1}
 
 static const icl_gemm_variant kTable[] = {
-    ICL_CGEMM_BF16X3_PACKED("cgemm_bf16x3d_mfma_256x128w8", 256, 128, 4, 2, 403.0f, 1),
-    ICL_CGEMM_BF16X3_PACKED("cgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, 327.4f, 1),
+    ICL_CGEMM_BF16X3_PACKED("cgemm_bf16x3d_mfma_256x128w8", 256, 128, 4, 2, 495.0f, 1),
+    ICL_CGEMM_BF16X3_PACKED("cgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, 387.0f, 1),
 };
 ICL_REGISTER_VARIANTS(kTable)

@@ -74,7 +74,8 @@ typedef struct icl_gemm_variant {
     int threads;
     hipError_t (*launch)(const icl_gemm_args* args, int opa, int opb, int edge, void* ws, hipStream_t stream);
     size_t (*workspace)(const icl_gemm_args* args);
-    float tflops_hint; /* measured useful TFLOP/s on a full MI355X (m = n = k = 8192, NN) */
+    float tflops_hint; /* measured useful TFLOP/s of the main kernel (a prepass is timed apart)
+                          on a full MI355X (m = n = k = 8192, NN) */
     int occupancy;     /* workgroups resident per CU */
     int level3;        /* implements the level-3 fields write_tri, kband, cflags */
 } icl_gemm_variant;

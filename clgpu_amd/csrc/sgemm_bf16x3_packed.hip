@@ -358,8 +358,8 @@ hipError_t packed_launch(const icl_gemm_args* g, int opa, int opb, int edge, voi
      1}
 
 static const icl_gemm_variant kTable[] = {
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8", 256, 256, 4, 2, 406.2f, 1),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8n", 256, 256, 2, 4, 417.4f, 1),
-    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, 344.6f, 2),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8", 256, 256, 4, 2, 458.0f, 1),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_256x256w8n", 256, 256, 2, 4, 463.0f, 1),
+    ICL_SGEMM_BF16X3_PACKED("sgemm_bf16x3d_mfma_128x128", 128, 128, 2, 2, 366.0f, 2),
 };
 ICL_REGISTER_VARIANTS(kTable)
