@@ -565,8 +565,8 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
         return;
     }
     // Panels of 256 unknowns: the inverse W_p of every diagonal panel is formed
-    // up front (one workgroup per panel, from the inverted 64 x 64 blocks; none
-    // depends on B), then panel by panel X_p = W_p B_p (left) or B_p W_p (right)
+    // up front (from the inverted 64 x 64 blocks, by recursive doubling in three
+    // launches over all panels; none depends on B), then panel by panel X_p = W_p B_p (left) or B_p W_p (right)
     // with the panel's right-hand sides already updated, and one CGEMM updates
     // every later unknown. W_p is stored in original coordinates: reversed in both
     // indices for a backward solve, transposed for a right-side solve
@@ -575,8 +575,9 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
     const int npanels = (t.order + kPanel - 1) / kPanel;
     const size_t inv_floats = (size_t)nblk * kLeaf * kLeaf * 2, x_floats = (size_t)m * n * 2;
     const size_t w_floats = (size_t)npanels * kPanel * kPanel * 2;
+    const size_t t_floats = (size_t)npanels * 4 * kLeaf * kLeaf * 2;
     float* ws = static_cast<float*>(h->scratch[iclblasContext::kSlotTrsmWorkspace].get(
-        (inv_floats + x_floats + w_floats) * sizeof(float), h->stream));
+        (inv_floats + x_floats + w_floats + t_floats) * sizeof(float), h->stream));
     float* Xb = ws + inv_floats;
     float* W = Xb + x_floats;
     ia.invD = ws;
@@ -592,6 +593,7 @@ void ctrsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
     pa.w_rs = t.left ? 1 : kPanel;
     pa.w_cs = t.left ? kPanel : 1;
     pa.rev = t.fwd ? 0 : 1;
+    pa.T = W + w_floats;
     hip_check(icl_ctrsm_invert_panels(&pa, h->stream), "ctrsm_invert_panels");
     for (int ps = 0; ps < t.order; ps += kPanel) {
         const int pe = std::min(t.order, ps + kPanel);

@@ -277,28 +277,82 @@ __global__ void __launch_bounds__(kApThreads) ctrsm_apply_kernel(icl_ctrsm_apply
     }
 }
 
-// Inverses of the 256 x 256 diagonal panels (see icl_ctrsm_invert_panels).
-// Workgroup (p, j) forms the 64-wide column block j of panel p's inverse: the
-// column blocks of a triangular inverse do not depend on each other
-// (M_ij = -inv(L_ii) sum_{k=j}^{i-1} L_ik M_kj reads only column block j), so
-// the four run side by side. Four waves per workgroup; each 64 x 64 complex
-// product runs on v_mfma_f32_32x32x2_f32 with the operands staged in LDS as
-// planar re / im tiles, A as [k][row] and B as [k][col], each wave owning a
-// 32 x 32 quadrant. Computed blocks go to W and are read back from there by the
-// later products; a fence at agent scope around the barrier makes them visible
-// across the workgroup.
+// Inverses of the 256 x 256 diagonal panels (see icl_ctrsm_invert_panels), by
+// recursive doubling over 64 x 64 blocks B_ij of a panel (0 <= i, j < 4), from
+// the inverted diagonal blocks iD_i: a lower triangular [A 0; C D] has the
+// inverse [inv(A) 0; -inv(D) C inv(A) inv(D)]. Three launches:
+//  level 0, workgroup (p, h): the 128 x 128 diagonal halves {2h, 2h + 1}:
+//    W_aa = iD_a, W_dd = iD_d, W_ad = 0, W_da = -iD_d (L_da iD_a);
+//    workgroup h = 0 also zeroes the upper right quadrant (W_02, W_03, W_12, W_13);
+//  level 1, workgroup (p, q): T_rc = sum_{k = c..1} L_rk W_kc (r = 2 + q / 2, c = q % 2),
+//    i.e. T = C inv(A), into a scratch array;
+//  level 2, workgroup (p, q): W_rc = -sum_{k = 2..r} W_rk T_kc, i.e. -inv(D) T.
+// The longest chain is two 64 x 64 x 64 complex products per launch, where
+// building the column blocks one after the other (M_ij = -iD_i sum_{k=j}^{i-1}
+// L_ik M_kj) chains nine. Each product runs on v_mfma_f32_32x32x2_f32 with the
+// operands staged in LDS as planar re / im tiles, A as [k][row] and B as
+// [k][col], each of the four waves owning a 32 x 32 quadrant.
 constexpr int kPanRows = 256;
 constexpr int kPnLd = kNB + 1;
 constexpr size_t kPanInvLds = sizeof(float) * 4 * kNB * kPnLd;
+constexpr int kPer = kNB * kNB / 256; // elements of a 64 x 64 block per thread; e = tid + 256 q: (x = e % 64, y = e / 64)
 
-__device__ __forceinline__ void publish()
-{
-    __threadfence();
-    __syncthreads();
-    __threadfence();
-}
+// A 64 x 64 operand block: element (x, y) of the block.
+struct PanBlk {
+    enum Kind { L, INVD, WBLK, TBLK } kind;
+    int bi, bj; // block coordinates within the panel (INVD: diagonal block bi; TBLK: T_(bi, bj))
+};
 
-__global__ void __launch_bounds__(256) ctrsm_invert_panels_kernel(icl_ctrsm_panel_inv_args a)
+struct PanCtx {
+    const icl_ctrsm_panel_inv_args& a;
+    int pnl, ps, plen;
+    float2* W;
+    float2* T;
+    __device__ long w_at(int r, int c) const
+    {
+        return a.rev ? (long)(plen - 1 - r) * a.w_rs + (long)(plen - 1 - c) * a.w_cs
+                     : (long)r * a.w_rs + (long)c * a.w_cs;
+    }
+    __device__ float2* t_blk(int r, int c) const { return T + ((long)pnl * 4 + (r - 2) * 2 + c) * kNB * kNB; }
+    // All loads first, unconditionally (an entry outside the system or the panel
+    // reads element 0 of its array instead and is replaced by 0 afterwards).
+    __device__ void load(const PanBlk& b, int tid, float2 (&v)[kPer]) const
+    {
+        const float2* L = reinterpret_cast<const float2*>(a.L);
+        const float2* invD = reinterpret_cast<const float2*>(a.invD);
+        const float sg = a.conj ? -1.0f : 1.0f;
+        bool ok[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + 256 * q, x = e % kNB, y = e / kNB;
+            const int r = kNB * b.bi + x, c = kNB * b.bj + y;
+            switch (b.kind) {
+            case PanBlk::L:
+                ok[q] = ps + r < a.nsys && ps + c < a.nsys;
+                v[q] = L[ok[q] ? (long)(ps + r) * a.l_si + (long)(ps + c) * a.l_sk : 0];
+                v[q].y *= sg;
+                break;
+            case PanBlk::INVD:
+                ok[q] = true;
+                v[q] = invD[((long)(ps / kNB + b.bi) * kNB + y) * kNB + x];
+                break;
+            case PanBlk::WBLK:
+                ok[q] = r < plen && c < plen;
+                v[q] = W[ok[q] ? w_at(r, c) : 0];
+                break;
+            default:
+                ok[q] = true;
+                v[q] = t_blk(b.bi, b.bj)[(long)y * kNB + x];
+                break;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+            if (!ok[q]) v[q] = make_float2(0.0f, 0.0f);
+    }
+};
+
+__global__ void __launch_bounds__(256) ctrsm_invert_panels_kernel(icl_ctrsm_panel_inv_args a, int level)
 {
     extern __shared__ float sm[];
     float(*Ar)[kPnLd] = reinterpret_cast<float(*)[kPnLd]>(sm); // A[k][row]
@@ -307,36 +361,27 @@ __global__ void __launch_bounds__(256) ctrsm_invert_panels_kernel(icl_ctrsm_pane
     float(*Bi)[kPnLd] = Br + kNB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
-    const int pnl = blockIdx.x, j = blockIdx.y, ps = pnl * kPanRows;
+    const int pnl = blockIdx.x, ps = pnl * kPanRows;
     const int plen = min(kPanRows, a.nsys - ps), nb = (plen + kNB - 1) / kNB;
-    if (j >= nb) return; // the whole workgroup leaves before any barrier
-    const float2* L = reinterpret_cast<const float2*>(a.L);
-    const float2* invD = reinterpret_cast<const float2*>(a.invD);
-    float2* W = reinterpret_cast<float2*>(a.W) + (long)pnl * kPanRows * kPanRows;
-    const float sg = a.conj ? -1.0f : 1.0f;
-    auto w_at = [&](int r, int c) -> long {
-        return a.rev ? (long)(plen - 1 - r) * a.w_rs + (long)(plen - 1 - c) * a.w_cs : (long)r * a.w_rs + (long)c * a.w_cs;
-    };
-    constexpr int PER = kNB * kNB / 256; // elements of a 64 x 64 block per thread; e = tid + 256 q: (row e % 64, col e / 64)
+    const PanCtx cx{a, pnl, ps, plen, reinterpret_cast<float2*>(a.W) + (long)pnl * kPanRows * kPanRows,
+                    reinterpret_cast<float2*>(a.T)};
 
-    // Blocks of column block j above the diagonal are 0, the diagonal one is inv(L_jj).
-    for (int bi = 0; bi <= j; ++bi) {
-        float2 v[PER];
+    // Stage X as the A tile and Y as the B tile (after a barrier, so no wave still
+    // reads the previous tiles), then acc (+)= X Y; lanes along rows of the result.
+    auto product = [&](const PanBlk& X, const PanBlk& Y, f32x16& accr, f32x16& acci) {
+        float2 va[kPer], vb[kPer];
+        cx.load(X, tid, va);
+        cx.load(Y, tid, vb);
+        __syncthreads();
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = tid + 256 * q, r = e % kNB, c = e / kNB;
-            v[q] = bi == j ? invD[((long)(ps / kNB + j) * kNB + c) * kNB + r] : make_float2(0.0f, 0.0f);
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + 256 * q, x = e % kNB, y = e / kNB; // X: (row x, k y); Y: (k x, col y)
+            Ar[y][x] = va[q].x;
+            Ai[y][x] = va[q].y;
+            Br[x][y] = vb[q].x;
+            Bi[x][y] = vb[q].y;
         }
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = tid + 256 * q, r = kNB * bi + e % kNB, c = kNB * j + e / kNB;
-            if (r < plen && c < plen) W[w_at(r, c)] = v[q];
-        }
-    }
-    publish();
-
-    // acc (+)= A B for the staged tiles; lanes along rows of the result.
-    auto mma = [&](f32x16& accr, f32x16& acci) {
+        __syncthreads();
 #pragma unroll 8
         for (int kk = 0; kk < kNB; kk += 2) {
             const int kr = kk + (lane >> 5);
@@ -348,64 +393,84 @@ __global__ void __launch_bounds__(256) ctrsm_invert_panels_kernel(icl_ctrsm_pane
             acci = icl::mfma_32x32x2_f32(br, ai, acci);
         }
     };
-    for (int i = j + 1; i < nb; ++i) {
-        // S = sum_{k=j}^{i-1} L_ik M_kj
-        f32x16 sr = f32x16{}, si = f32x16{};
-        for (int k = j; k < i; ++k) {
-            // All loads first, unconditionally (an entry outside the system reads
-            // element 0 of L or of the panel's W instead and is replaced by 0),
-            // so that they are in flight together.
-            float2 va[PER], vb[PER];
-#pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const int e = tid + 256 * q, x = e % kNB, y = e / kNB; // A: (row x, k y); B: (k x, col y)
-                const int lr = ps + kNB * i + x, lk = ps + kNB * k + y;
-                va[q] = L[(lr < a.nsys && lk < a.nsys) ? (long)lr * a.l_si + (long)lk * a.l_sk : 0];
-                const int mr = kNB * k + x, mc = kNB * j + y;
-                vb[q] = W[(mr < plen && mc < plen) ? w_at(mr, mc) : 0];
-            }
-#pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const int e = tid + 256 * q, x = e % kNB, y = e / kNB;
-                const bool lin = ps + kNB * i + x < a.nsys && ps + kNB * k + y < a.nsys;
-                const bool win = kNB * k + x < plen && kNB * j + y < plen;
-                Ar[y][x] = lin ? va[q].x : 0.0f;
-                Ai[y][x] = lin ? sg * va[q].y : 0.0f;
-                Br[x][y] = win ? vb[q].x : 0.0f;
-                Bi[x][y] = win ? vb[q].y : 0.0f;
-            }
-            __syncthreads();
-            mma(sr, si);
-            __syncthreads();
-        }
-        // M_ij = -inv(L_ii) S: S becomes the B tile, inv(L_ii) the A tile.
-        float2 vd[PER];
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = tid + 256 * q, x = e % kNB, y = e / kNB; // (row x, k y)
-            vd[q] = invD[((long)(ps / kNB + i) * kNB + y) * kNB + x];
-        }
+    // Element (row, col) of this lane's accumulator entry r.
+    auto row_of = [&](int) { return wr + (lane & 31); };
+    auto col_of = [&](int r) { return wc + icl::mfma32_row(lane, r); };
+    auto store_w = [&](int bi, int bj, const f32x16& accr, const f32x16& acci, float s) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            Br[wr + (lane & 31)][wc + icl::mfma32_row(lane, r)] = sr[r];
-            Bi[wr + (lane & 31)][wc + icl::mfma32_row(lane, r)] = si[r];
+            const int row = kNB * bi + row_of(r), col = kNB * bj + col_of(r);
+            if (row < plen && col < plen) cx.W[cx.w_at(row, col)] = make_float2(s * accr[r], s * acci[r]);
+        }
+    };
+    auto copy_w = [&](const PanBlk& src, int bi, int bj) { // W_(bi, bj) = src (zero when src is null)
+        float2 v[kPer];
+        if (src.kind == PanBlk::INVD) cx.load(src, tid, v);
+        else
+            for (int q = 0; q < kPer; ++q) v[q] = make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + 256 * q, r = kNB * bi + e % kNB, c = kNB * bj + e / kNB;
+            if (r < plen && c < plen) cx.W[cx.w_at(r, c)] = v[q];
+        }
+    };
+    const PanBlk zero{PanBlk::WBLK, -1, -1};
+
+    if (level == 0) {
+        const int h = blockIdx.y, da = 2 * h, dd = da + 1;
+        if (da >= nb) return;
+        copy_w({PanBlk::INVD, da, da}, da, da);
+        copy_w(zero, da, dd);
+        if (h == 0)
+            for (int i = 0; i < 2; ++i)
+                for (int j = 2; j < 4; ++j) copy_w(zero, i, j);
+        if (dd >= nb) return;
+        copy_w({PanBlk::INVD, dd, dd}, dd, dd);
+        f32x16 sr = f32x16{}, si = f32x16{};
+        product({PanBlk::L, dd, da}, {PanBlk::INVD, da, da}, sr, si); // S = L_da iD_a
+        // W_da = -iD_d S: S becomes the B tile, iD_d the A tile.
+        float2 vd[kPer];
+        cx.load({PanBlk::INVD, dd, dd}, tid, vd);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            Br[row_of(r)][col_of(r)] = sr[r];
+            Bi[row_of(r)][col_of(r)] = si[r];
         }
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
+        for (int q = 0; q < kPer; ++q) {
             const int e = tid + 256 * q, x = e % kNB, y = e / kNB;
             Ar[y][x] = vd[q].x;
             Ai[y][x] = vd[q].y;
         }
         __syncthreads();
         f32x16 mr = f32x16{}, mi = f32x16{};
-        mma(mr, mi);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = kNB * i + wr + (lane & 31), col = kNB * j + wc + icl::mfma32_row(lane, r);
-            if (row < plen && col < plen) W[w_at(row, col)] = make_float2(-mr[r], -mi[r]);
+#pragma unroll 8
+        for (int kk = 0; kk < kNB; kk += 2) {
+            const int kr = kk + (lane >> 5);
+            const float ar = Ar[kr][wr + (lane & 31)], ai = Ai[kr][wr + (lane & 31)];
+            const float br = Br[kr][wc + (lane & 31)], bi = Bi[kr][wc + (lane & 31)];
+            mr = icl::mfma_32x32x2_f32(br, ar, mr);
+            mi = icl::mfma_32x32x2_f32(bi, ar, mi);
+            mr = icl::mfma_32x32x2_f32(bi, -ai, mr);
+            mi = icl::mfma_32x32x2_f32(br, ai, mi);
         }
-        publish();
+        store_w(dd, da, mr, mi, -1.0f);
+        return;
     }
+    const int q = blockIdx.y, r = 2 + (q >> 1), c = q & 1;
+    if (r >= nb) return;
+    f32x16 accr = f32x16{}, acci = f32x16{};
+    if (level == 1) { // T_rc = sum_{k = c..1} L_rk W_kc
+        for (int k = c; k < 2; ++k) product({PanBlk::L, r, k}, {PanBlk::WBLK, k, c}, accr, acci);
+        float2* t = cx.t_blk(r, c);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) t[(long)col_of(e) * kNB + row_of(e)] = make_float2(accr[e], acci[e]);
+        return;
+    }
+    // level 2: W_rc = -sum_{k = 2..r} W_rk T_kc
+    for (int k = 2; k <= r; ++k) product({PanBlk::WBLK, r, k}, {PanBlk::TBLK, k, c}, accr, acci);
+    store_w(r, c, accr, acci, -1.0f);
 }
 
 } // namespace
@@ -417,8 +482,13 @@ extern "C" hipError_t icl_ctrsm_invert_panels(const icl_ctrsm_panel_inv_args* a,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPanInvLds);
     if (attr != hipSuccess) return attr;
     const unsigned panels = (unsigned)((a->nsys + kPanRows - 1) / kPanRows);
-    hipLaunchKernelGGL(ctrsm_invert_panels_kernel, dim3(panels, kPanRows / kNB), dim3(256), kPanInvLds, stream, *a);
-    return hipGetLastError();
+    for (int level = 0; level < 3; ++level) {
+        hipLaunchKernelGGL(ctrsm_invert_panels_kernel, dim3(panels, level == 0 ? 2 : 4), dim3(256), kPanInvLds, stream,
+                           *a, level);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 extern "C" hipError_t icl_ctrsm_invert_blocks(const icl_ctrsm_blocked_args* a, hipStream_t stream)

@@ -221,6 +221,7 @@ typedef struct icl_ctrsm_panel_inv_args {
     float* W;
     long w_rs, w_cs;
     int rev;
+    float* T; /* scratch: 4 blocks of 64 x 64 complex per panel */
 } icl_ctrsm_panel_inv_args;
 hipError_t icl_ctrsm_invert_panels(const icl_ctrsm_panel_inv_args* a, hipStream_t stream);
 

@@ -57,7 +57,7 @@ def _problem(side, uplo, diag, m, n, seed=41, pad=3):
 
 @pytest.mark.parametrize("impl", [None, "ctrsm_recursive_inv64"], ids=["default", "recursive"])
 @pytest.mark.parametrize("v", VARIANTS, ids=vid)
-@pytest.mark.parametrize("shape", [(5, 3), (64, 64), (65, 17), (200, 130), (600, 40), (70, 520), (520, 70)],
+@pytest.mark.parametrize("shape", [(5, 3), (64, 64), (65, 17), (200, 130), (600, 40), (70, 520), (520, 70), (700, 33)],
                          ids=lambda s: f"{s[0]}x{s[1]}")
 def test_ctrsm_variants_vs_fp64(h, v, shape, impl):
     side, uplo, trans, diag = v
