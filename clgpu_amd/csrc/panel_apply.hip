@@ -8,7 +8,13 @@
 // B(k, l0 + lane % 32) for k = k0 + 2 j + lane / 32, j < 32 -- all loads issued
 // before the first product, so the operands never pass through LDS (lanes that
 // share a cache line across the j steps hit it in L1); the four partial blocks
-// are added through LDS.
+// are added through LDS. When an operand of a complex product has k as its
+// contiguous dimension (WV: W, BV: B), its lanes read G = 2 or 4 consecutive k
+// (16 or 32 bytes), and both operands take step j at
+// k = k0 + 2 G (j / G) + G h + j % G (h = lane / 32) instead: the same sum in
+// another order, with 1 / (2 G) of the cache lines per load instruction.
+#include <cstdint>
+
 #include "mfma.hpp"
 #include "kernel_abi.h"
 
@@ -25,9 +31,11 @@ struct PanelCfg {
     static constexpr size_t LDS = sizeof(float) * 4 * PART_FLOATS;
 };
 
-template <bool CPLX>
+template <bool CPLX, int G = 1, bool WV = false, bool BV = false>
 __global__ void __launch_bounds__(256) panel_apply_kernel(icl_panel_apply_args a)
 {
+    static_assert(CPLX || G == 1, "grouped loads are for complex operands");
+    static_assert((G == 1) == !(WV || BV) && (G == 1 || G == 2 || G == 4), "G > 1 exactly with a grouped operand");
     using Cfg = PanelCfg<CPLX>;
     extern __shared__ float sm[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -36,24 +44,48 @@ __global__ void __launch_bounds__(256) panel_apply_kernel(icl_panel_apply_args a
     const E* W = reinterpret_cast<const E*>(a.W);
     const E* B = reinterpret_cast<const E*>(a.B);
 
-    // Operands of step j (k = k0 + 2 j + h): W(r, k) for this lane's row r and
-    // B(k, l) for its line l; out-of-range values read as 0.
+    // Operands of step j (k = kof(j)): W(r, k) for this lane's row r and
+    // B(k, l) for its line l; out-of-range values read as 0. A grouped load reads
+    // k .. k + G - 1 (k a multiple of G, and so is p: all in range or none).
     const int c = lane & 31, h = lane >> 5;
     const int r = r0 + c, l = l0 + c;
     const bool r_ok = r < a.p, l_ok = l < a.nlines;
     const E* wp = W + (r_ok ? (long)r * a.w_rs : 0);
     const E* bp = B + (l_ok ? (long)l * a.b_ls : 0);
+    auto kof = [&](int j) { return k0 + 2 * G * (j / G) + G * h + j % G; };
     E wv[kPK / 2], bv[kPK / 2];
 #pragma unroll
     for (int j = 0; j < kPK / 2; ++j) {
-        const int k = k0 + 2 * j + h;
+        const int k = kof(j);
         const bool k_ok = k < a.p;
-        wv[j] = wp[r_ok && k_ok ? (long)k * a.w_ks : 0];
-        bv[j] = bp[l_ok && k_ok ? (long)k * a.b_ks : 0];
+        if constexpr (WV) {
+            if (j % G == 0) {
+#pragma unroll
+                for (int g = 0; g < G; g += 2) {
+                    const float4 v = *reinterpret_cast<const float4*>(wp + (r_ok && k_ok ? k + g : g));
+                    wv[j + g] = make_float2(v.x, v.y);
+                    wv[j + g + 1] = make_float2(v.z, v.w);
+                }
+            }
+        } else {
+            wv[j] = wp[r_ok && k_ok ? (long)k * a.w_ks : 0];
+        }
+        if constexpr (BV) {
+            if (j % G == 0) {
+#pragma unroll
+                for (int g = 0; g < G; g += 2) {
+                    const float4 v = *reinterpret_cast<const float4*>(bp + (l_ok && k_ok ? k + g : g));
+                    bv[j + g] = make_float2(v.x, v.y);
+                    bv[j + g + 1] = make_float2(v.z, v.w);
+                }
+            }
+        } else {
+            bv[j] = bp[l_ok && k_ok ? (long)k * a.b_ks : 0];
+        }
     }
 #pragma unroll
     for (int j = 0; j < kPK / 2; ++j) {
-        const bool k_ok = k0 + 2 * j + h < a.p;
+        const bool k_ok = kof(j) < a.p;
         if (!(r_ok && k_ok)) wv[j] = E{};
         if (!(l_ok && k_ok)) bv[j] = E{};
     }
@@ -108,16 +140,23 @@ extern "C" hipError_t icl_panel_apply(const icl_panel_apply_args* a, hipStream_t
     if (a->p > kPanelMax) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((a->nlines + kPL - 1) / kPL), (unsigned)((a->p + kPR - 1) / kPR));
     if (a->cplx) {
-        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(panel_apply_kernel<true>),
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                           (int)PanelCfg<true>::LDS);
-        if (attr != hipSuccess) return attr;
-        hipLaunchKernelGGL(panel_apply_kernel<true>, grid, dim3(256), PanelCfg<true>::LDS, stream, *a);
+        // Groups of G consecutive k in 16-byte loads: k contiguous, p a multiple
+        // of G (groups never straddle the end), 16-byte aligned rows / lines.
+        auto a16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+        const int G = a->p % 4 == 0 ? 4 : a->p % 2 == 0 ? 2 : 1;
+        const bool wv = G > 1 && a->w_ks == 1 && a->w_rs % 2 == 0 && a16(a->W);
+        const bool bv = G > 1 && a->b_ks == 1 && a->b_ls % 2 == 0 && a16(a->B);
+        using K = void (*)(icl_panel_apply_args);
+        static const K table[2][2][2] = { // [G == 4][wv][bv]
+            {{panel_apply_kernel<true>, panel_apply_kernel<true, 2, false, true>},
+             {panel_apply_kernel<true, 2, true, false>, panel_apply_kernel<true, 2, true, true>}},
+            {{panel_apply_kernel<true>, panel_apply_kernel<true, 4, false, true>},
+             {panel_apply_kernel<true, 4, true, false>, panel_apply_kernel<true, 4, true, true>}}};
+        const void* k = reinterpret_cast<const void*>(table[G == 4][wv][bv]);
+        icl_panel_apply_args args = *a;
+        void* params[] = {&args};
+        return hipLaunchKernel(k, grid, dim3(256), params, PanelCfg<true>::LDS, stream);
     } else {
-        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(panel_apply_kernel<false>),
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                           (int)PanelCfg<false>::LDS);
-        if (attr != hipSuccess) return attr;
         hipLaunchKernelGGL(panel_apply_kernel<false>, grid, dim3(256), PanelCfg<false>::LDS, stream, *a);
     }
     return hipGetLastError();
