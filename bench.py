@@ -719,7 +719,7 @@ def sgemm_line(a, h, ib, stream, dev):
     def traffic_of(impl):
         r = traffic_rec.get(impl)
         if isinstance(r, dict) and r.get("workload") == f"SGEMM {a.trans} {n}^3":
-            return r.get("bytes_per_launch")
+            return r.get("bytes_per_call")
         return None
     traffic = traffic_of(hr["implementation"])
     line = {
@@ -741,10 +741,13 @@ def sgemm_line(a, h, ib, stream, dev):
         "roofline": {"bound": "mfma", "achieved": hr["executed_tflops"], "peak": hr["mfma_peak"],
                      "unit": "TFLOP/s", "frac": hr["mfma_frac"], "traffic": traffic,
                      "kernel": hr["implementation"], "kernel_ms": hr["kernel_ms"],
-                     "note": "achieved = executed MFMA flops per launch (2mnk; 3 x 2mnk for split-BF16) / mean "
-                             "duration of the main GEMM kernel (HIP events around it, same timed region); traffic "
-                             "= HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes of "
-                             "profiles/sgemm_8192_hbm_traffic.md (profiles/hbm_traffic.json)"},
+                     "note": "achieved = executed MFMA flops of the call's product (2mnk; 3 x 2mnk for "
+                             "split-BF16) / mean duration of the HIP-event pair around it (same timed region): the "
+                             "one FP32 kernel; for split-BF16 at 8192^2 the three partial-product launches, one of "
+                             "them beside the side-stream pack of the operands' second halves (the first halves' "
+                             "pack is outside the pair); traffic = HBM bytes of those launches per call from the "
+                             "rocprofv3 FETCH_SIZE / WRITE_SIZE passes of profiles/sgemm_8192_hbm_traffic.md "
+                             "(profiles/hbm_traffic.json)"},
     }
     if "fp32" in results:
         r = results["fp32"]
