@@ -231,6 +231,27 @@ bool split_depth(iclblasContext* h, int k)
     return h->math_mode == ICLBLAS_MATH_SPLIT_BF16X3 && k >= kSplitMinK;
 }
 
+// A square triangle update (SYRK type) as two launches -- the tiles off the
+// diagonal 256 x 256 blocks, which lie wholly inside the triangle and take the
+// largest tile, then the diagonal band on smaller tiles -- when the scoring
+// model puts the pair at least 3% below the whole triangle on one tile size
+// (SSYRK 8192: 2080 tiles of 128^2 end in a 32-tile round; the 496 off-band
+// tiles of 256^2 fill two rounds). FP32 path only: in the split-BF16 mode each
+// launch would pack the operands again. Not with a forced implementation.
+static bool band_split_pays(iclblasContext* h, int kind, int math, int opa, int opb, const icl_gemm_args& g)
+{
+    if (!h->forced_impl.empty() || math != ICL_MATH_FP32 || g.kband != ICL_KBAND_NONE || g.M != g.N ||
+        g.M < 2 * ICL_TRI_BAND_BLOCK || (g.write_tri != ICL_TRI_LOWER && g.write_tri != ICL_TRI_UPPER))
+        return false;
+    auto est = [&](int tri) {
+        icl_gemm_args t = g;
+        t.write_tri = tri;
+        const GemmProblem p{kind, math, opa, opb, &t};
+        return 1.0 / gemm_score(*gemm_select(h, p), p);
+    };
+    return est(g.write_tri | ICL_TRI_OFF_BAND) + est(g.write_tri | ICL_TRI_BAND) < 0.97 * est(g.write_tri);
+}
+
 void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n, int k, const float alpha[2],
                      const float* A, long lda, const float* B, long ldb, const float beta[2], float* C, long ldc,
                      int write_tri, int kband, int cflags)
@@ -253,7 +274,13 @@ void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n
     g.write_tri = write_tri;
     g.kband = kband;
     g.cflags = cflags;
-    run_gemm(h, kind, split_depth(h, k) ? ICL_MATH_BF16X3 : ICL_MATH_FP32, opa, opb, g, true);
+    const int math = split_depth(h, k) ? ICL_MATH_BF16X3 : ICL_MATH_FP32;
+    if (band_split_pays(h, kind, math, opa, opb, g)) {
+        g.write_tri = write_tri | ICL_TRI_OFF_BAND;
+        run_gemm(h, kind, math, opa, opb, g, true);
+        g.write_tri = write_tri | ICL_TRI_BAND;
+    }
+    run_gemm(h, kind, math, opa, opb, g, true);
 }
 
 // ---------------------------------------------------------------------------

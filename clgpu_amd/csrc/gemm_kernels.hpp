@@ -53,8 +53,8 @@ struct TriMask {
 template <int BM, int BN>
 __device__ __forceinline__ bool tile_window(const GemmArgs& g, int m0, int n0, int& kb, int& ke)
 {
-    if (g.write_tri == ICL_TRI_LOWER && m0 + BM - 1 < n0) return false;
-    if (g.write_tri == ICL_TRI_UPPER && m0 > n0 + BN - 1) return false;
+    if ((g.write_tri & 3) == ICL_TRI_LOWER && m0 + BM - 1 < n0) return false;
+    if ((g.write_tri & 3) == ICL_TRI_UPPER && m0 > n0 + BN - 1) return false;
     // Both bounds as values (see tile_coords_level3 on stores per case).
     const int kband = g.kband;
     kb = kband == ICL_KBAND_K_FROM_M0 ? min(g.K, m0) : kband == ICL_KBAND_K_FROM_N0 ? min(g.K, n0) : 0;
@@ -64,10 +64,11 @@ __device__ __forceinline__ bool tile_window(const GemmArgs& g, int m0, int n0, i
     return true;
 }
 
-// Store predicate of g.write_tri.
+// Store predicate of g.write_tri (its band flags select tiles, not elements).
 __device__ __forceinline__ bool tri_writes(int write_tri, int m, int n)
 {
-    return write_tri == ICL_TRI_NONE || (write_tri == ICL_TRI_LOWER ? m >= n : m <= n);
+    const int tri = write_tri & 3;
+    return tri == ICL_TRI_NONE || (tri == ICL_TRI_LOWER ? m >= n : m <= n);
 }
 
 // ---------------------------------------------------------------------------

@@ -33,7 +33,15 @@ typedef struct icl_gemm_args {
 enum {
     ICL_TRI_NONE = 0,
     ICL_TRI_LOWER = 1, /* m >= n */
-    ICL_TRI_UPPER = 2  /* m <= n */
+    ICL_TRI_UPPER = 2, /* m <= n */
+    /* Flags on LOWER / UPPER, splitting a square triangle at its diagonal blocks
+     * of ICL_TRI_BAND_BLOCK: OFF_BAND launches only the tiles outside them (wholly
+     * inside the triangle), BAND only the tiles inside them; the two launches
+     * write disjoint parts that together make the triangle. Tile sizes must
+     * divide ICL_TRI_BAND_BLOCK. */
+    ICL_TRI_OFF_BAND = 4,
+    ICL_TRI_BAND = 8,
+    ICL_TRI_BAND_BLOCK = 256
 };
 
 /* k window of a tile at rows [m0, m0 + BM), columns [n0, n0 + BN): */

@@ -281,3 +281,53 @@ def test_level3_large_device(dev_handle, routine, v, shape):
     scale = np.abs(exp[:rc, :cc]).max()
     err = np.abs(got[:rc, :cc] - exp[:rc, :cc]).max()
     assert err <= 4 * kk * U * max(scale, 1.0) * 4, (err, scale)
+
+
+@pytest.mark.parametrize("routine", ["Ssyrk", "Ssyr2k"])
+@pytest.mark.parametrize("uplo,trans", [(ib.FILL_MODE_LOWER, ib.OP_N), (ib.FILL_MODE_UPPER, ib.OP_T)],
+                         ids=["LN", "UT"])
+def test_syrk_band_split_8192(dev_handle, routine, uplo, trans):
+    """SSYRK / SSYR2K at n = 8192, k = 2048 on the FP32 path, where the selection
+    runs each product as two launches (the tiles off the diagonal 256 x 256
+    blocks, then the diagonal band): two timed launches per product, the written
+    triangle against a float64 product on the GPU within the elementwise bound,
+    the other triangle bit for bit untouched."""
+    torch = _torch()
+    n, k, alpha, beta = 8192, 2048, 0.75, -0.5
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    rows, cols = (n, k) if trans == ib.OP_N else (k, n)
+    A = torch.rand((cols, rows), generator=g, device="cuda") * 2 - 1  # column-major rows x cols, ld = rows
+    B = torch.rand((cols, rows), generator=g, device="cuda") * 2 - 1
+    C0 = torch.rand((n, n), generator=g, device="cuda") * 2 - 1
+    C = C0.clone()
+    h = dev_handle
+    h.set_math_mode(ib.MATH_FP32)
+    h.set_implementation(None)
+    h.set_kernel_timing(True)
+    if routine == "Ssyrk":
+        st = ib.iclblasSsyrk(h.value, uplo, trans, n, k, alpha, A.data_ptr(), rows, beta, C.data_ptr(), n)
+    else:
+        st = ib.iclblasSsyr2k(h.value, uplo, trans, n, k, alpha, A.data_ptr(), rows, B.data_ptr(), rows, beta,
+                              C.data_ptr(), n)
+    ib.check(st, routine)
+    torch.cuda.synchronize()
+    _, launches = h.kernel_timing()
+    h.set_kernel_timing(False)
+    assert launches == (2 if routine == "Ssyrk" else 4)
+    opA = A.double().t() if trans == ib.OP_N else A.double()  # n x k
+    opB = B.double().t() if trans == ib.OP_N else B.double()
+    if routine == "Ssyrk":
+        P, mag, kk = opA @ opA.t(), opA.abs() @ opA.abs().t(), k
+    else:
+        P = opA @ opB.t() + opB @ opA.t()
+        mag, kk = opA.abs() @ opB.abs().t() + opB.abs() @ opA.abs().t(), 2 * k
+    ref = alpha * P + beta * C0.double().t()
+    del P
+    keep = torch.ones((n, n), dtype=torch.bool, device="cuda")
+    keep = keep.tril() if uplo == ib.FILL_MODE_LOWER else keep.triu()
+    got = C.t()
+    bound = 4.0 * (kk + 2) * U * (abs(alpha) * mag + abs(beta) * C0.double().t().abs())
+    err = (got.double() - ref).abs()
+    assert bool((err[keep] <= bound[keep]).all()), float((err[keep] / bound[keep]).max())
+    assert torch.equal(got[~keep], C0.t()[~keep])
