@@ -815,6 +815,31 @@ def row_checks(line: dict):
     return out
 
 
+def device_info() -> dict:
+    """The gfx950 agent's name, compute units and peak engine clock from rocminfo
+    (a child process, run before this process touches the GPU), and the MFMA
+    peaks they give: 256 FP32 and 4096 dense BF16 flops per clock per CU."""
+    import re
+    import subprocess
+    try:
+        out = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=60).stdout
+    except (OSError, subprocess.SubprocessError):
+        return {}
+    for block in re.split(r"\n\s*Agent \d+", out)[1:]:
+        if "gfx950" not in block:
+            continue
+        cus = re.search(r"Compute Unit:\s+(\d+)", block)
+        mhz = re.search(r"Max Clock Freq\. \(MHz\):\s+(\d+)", block)
+        name = re.search(r"Marketing Name:\s+(.+)", block)
+        if not (cus and mhz):
+            continue
+        c, f = int(cus.group(1)), int(mhz.group(1))
+        return {"name": name.group(1).strip() if name else "", "arch": "gfx950", "compute_units": c,
+                "max_engine_clock_mhz": f, "fp32_mfma_peak_tflops": c * 256 * f * 1e6 / 1e12,
+                "bf16_mfma_peak_tflops": c * 4096 * f * 1e6 / 1e12}
+    return {}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -826,6 +851,7 @@ def main():
     if unknown:
         print(f"bench.py: unknown rows {sorted(unknown)}", file=sys.stderr)
         sys.exit(2)
+    device = device_info()
     import torch
 
     from clgpu_amd import iclblas as ib
@@ -880,6 +906,10 @@ def main():
         other["host_pointer"] = host_rows(ib, a.steps)
     if other:
         line["other_rows"] = other
+    if device:
+        # The roofline peaks above are the published MI355X figures (2.4 GHz, 256 CUs);
+        # these are the same peaks from this box's own agent properties.
+        line["device"] = device
     if not a.no_cpu_baseline and A is not None:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(A, B, a.size, a.cpu_seconds)

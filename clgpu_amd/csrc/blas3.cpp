@@ -38,7 +38,10 @@ static bool level3_problem(const icl_gemm_args& g)
 
 bool gemm_accept(const icl_gemm_variant& v, const GemmProblem& p)
 {
-    return v.kind == p.kind && v.math == p.math && (v.level3 || !level3_problem(*p.args));
+    // The band flags of write_tri select tiles by whole 256 x 256 diagonal blocks.
+    const bool band_ok = (p.args->write_tri & (ICL_TRI_OFF_BAND | ICL_TRI_BAND)) == 0 ||
+                         (ICL_TRI_BAND_BLOCK % v.bm == 0 && ICL_TRI_BAND_BLOCK % v.bn == 0);
+    return v.kind == p.kind && v.math == p.math && (v.level3 || !level3_problem(*p.args)) && band_ok;
 }
 
 static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
