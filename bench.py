@@ -828,13 +828,13 @@ def device_info() -> dict:
     for block in re.split(r"\n\s*Agent \d+", out)[1:]:
         if "gfx950" not in block:
             continue
-        cus = re.search(r"Compute Unit:\s+(\d+)", block)
-        mhz = re.search(r"Max Clock Freq\. \(MHz\):\s+(\d+)", block)
-        name = re.search(r"Marketing Name:\s+(.+)", block)
+        cus = re.search(r"Compute Unit:[ \t]+(\d+)", block)
+        mhz = re.search(r"Max Clock Freq\. \(MHz\):[ \t]+(\d+)", block)
+        name = re.search(r"Marketing Name:[ \t]*(.*)", block)
         if not (cus and mhz):
             continue
         c, f = int(cus.group(1)), int(mhz.group(1))
-        return {"name": name.group(1).strip() if name else "", "arch": "gfx950", "compute_units": c,
+        return {"name": (name.group(1).strip() if name else "") or "gfx950", "arch": "gfx950", "compute_units": c,
                 "max_engine_clock_mhz": f, "fp32_mfma_peak_tflops": c * 256 * f * 1e6 / 1e12,
                 "bf16_mfma_peak_tflops": c * 4096 * f * 1e6 / 1e12}
     return {}
