@@ -823,21 +823,20 @@ void strsm_dev(iclblasContext* h, int side, int uplo, int trans, int diag, int m
                 // W_p (256 x 256, column-major) such that the panel's unknowns in
                 // original order are X_p = W_p B_p (left) or B_p W_p (right): the
                 // canonical inverse inv_c, written reversed in both indices for a
-                // backward solve and transposed for a right-side solve, through the
-                // X view of a batched panel solve of L_pp Y = I.
-                hip_check(icl_identity_stack(E, order, kPanel, h->stream), "identity_stack");
-                icl_trsm_blocked_args ai = a;
-                ai.nlines = kPanel;
-                ai.B = E;
-                ai.x_si = 1;
-                ai.x_sj = order;
-                const long row_s = t.left ? 1 : kPanel, col_s = t.left ? kPanel : 1; // canonical (row, line) strides
-                const long last = (long)(kPanel - 1) * (row_s + col_s);
-                ai.X = t.fwd ? W : W + last;
-                ai.xv_si = t.fwd ? row_s : -row_s;
-                ai.xv_sj = t.fwd ? col_s : -col_s;
-                ai.x_batch = (long)kPanel * kPanel;
-                hip_check(icl_trsm_panel_batch(&ai, nfull, h->stream), "trsm_panel_batch");
+                // backward solve and transposed for a right-side solve, formed by
+                // recursive doubling from the inverted 64 x 64 blocks.
+                icl_trsm_panel_inv_args pa{};
+                pa.npanels = nfull;
+                pa.L = a.L;
+                pa.l_si = a.l_si;
+                pa.l_sk = a.l_sk;
+                pa.invD = a.invD;
+                pa.W = W;
+                pa.w_rs = t.left ? 1 : kPanel; // canonical (row, column) strides
+                pa.w_cs = t.left ? kPanel : 1;
+                pa.rev = t.fwd ? 0 : 1;
+                pa.T = E; // scratch: npanels * 16384 floats of the identity / M_p region
+                hip_check(icl_trsm_invert_panels(&pa, h->stream), "trsm_invert_panels");
             }
             // op(A)(i, j) = A[i * si + j * sk] in original coordinates
             auto opa = [&](int i0, int j0) { return A + (long)i0 * t.si + (long)j0 * t.sk; };

@@ -301,6 +301,24 @@ hipError_t icl_trsm_panel_batch(const icl_trsm_blocked_args* a, int npanels, hip
 /* E (nsys x width, column-major, ld nsys) = a vertical stack of width x width
  * identity blocks: E(r, l) = 1 if r % width == l, else 0. */
 hipError_t icl_identity_stack(float* E, int nsys, int width, hipStream_t stream);
+
+/* Inverses W_p of the npanels full 256 x 256 diagonal panels of the canonical
+ * lower-triangular L (L(i, k) = L[i * l_si + k * l_sk]) from its inverted 64 x 64
+ * diagonal blocks invD (icl_trsm_invert_blocks; block b at invD + 4096 b,
+ * element (r, c) at [c * 64 + r]): panel p's inverse element (r, c) goes to
+ * W[p * 65536 + r' * w_rs + c' * w_cs] with (r', c') = (r, c), or
+ * (255 - r, 255 - c) when rev. T is scratch of npanels * 16384 floats. */
+typedef struct icl_trsm_panel_inv_args {
+    int npanels;
+    const float* L;
+    long l_si, l_sk;
+    const float* invD;
+    float* W;
+    long w_rs, w_cs;
+    int rev;
+    float* T;
+} icl_trsm_panel_inv_args;
+hipError_t icl_trsm_invert_panels(const icl_trsm_panel_inv_args* a, hipStream_t stream);
 hipError_t icl_trsm_dataflow(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream);
 
 /* R = fp32(alpha * B0 - P * Q) with the products and the sum in FP64.

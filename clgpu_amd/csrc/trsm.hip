@@ -1272,6 +1272,147 @@ extern "C" hipError_t icl_trsm_panel(const icl_trsm_blocked_args* a, int ps, int
     return launch_panel(a, ps, pe, 1, stream);
 }
 
+namespace {
+
+// Inverses of the full 256 x 256 diagonal panels by recursive doubling over
+// their 64 x 64 blocks, from the inverted diagonal blocks iD_i: a lower
+// triangular [A 0; C D] has the inverse [inv(A) 0; -inv(D) C inv(A) inv(D)].
+// Three launches: level 0, workgroup (p, h), the 128 x 128 halves {2h, 2h + 1}
+// (W_aa = iD_a, W_dd = iD_d, W_ad = 0, W_da = -iD_d (L_da iD_a); h = 0 also zeroes
+// the upper right quadrant); level 1, workgroup (p, q), T_rc = sum_{k = c..1}
+// L_rk W_kc (r = 2 + q / 2, c = q % 2); level 2, W_rc = -sum_{k = 2..r} W_rk T_kc.
+// Each 64^3 product runs on v_mfma_f32_32x32x2_f32 with A staged as [k][row] and
+// B as [k][col] in LDS, each of the four waves owning a 32 x 32 quadrant.
+constexpr int kTpLd = kNB + 1;
+constexpr size_t kTpLds = sizeof(float) * 2 * kNB * kTpLd;
+constexpr int kTpPer = kNB * kNB / 256; // block elements per thread: e = tid + 256 q, (x = e % 64, y = e / 64)
+
+struct TpBlk {
+    enum Kind { L, INVD, WBLK, TBLK } kind;
+    int bi, bj;
+};
+
+__global__ void __launch_bounds__(256) trsm_invert_panels_kernel(icl_trsm_panel_inv_args a, int level)
+{
+    __shared__ float As[kNB][kTpLd]; // A[k][row]
+    __shared__ float Bs[kNB][kTpLd]; // B[k][col]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
+    const int pnl = blockIdx.x, ps = pnl * 256;
+    float* W = a.W + (long)pnl * 65536;
+    auto w_at = [&](int r, int c) -> long {
+        return a.rev ? (long)(255 - r) * a.w_rs + (long)(255 - c) * a.w_cs : (long)r * a.w_rs + (long)c * a.w_cs;
+    };
+    auto t_blk = [&](int r, int c) { return a.T + ((long)pnl * 4 + (r - 2) * 2 + c) * kNB * kNB; };
+    auto load = [&](const TpBlk& b, float (&v)[kTpPer]) {
+#pragma unroll
+        for (int q = 0; q < kTpPer; ++q) {
+            const int e = tid + 256 * q, x = e % kNB, y = e / kNB;
+            const int r = kNB * b.bi + x, c = kNB * b.bj + y;
+            switch (b.kind) {
+            case TpBlk::L: v[q] = a.L[(long)(ps + r) * a.l_si + (long)(ps + c) * a.l_sk]; break;
+            case TpBlk::INVD: v[q] = a.invD[((long)(ps / kNB + b.bi) * kNB + y) * kNB + x]; break;
+            case TpBlk::WBLK: v[q] = W[w_at(r, c)]; break;
+            default: v[q] = t_blk(b.bi, b.bj)[(long)y * kNB + x]; break;
+            }
+        }
+    };
+    // acc += X Y with X staged as the A tile and Y as the B tile (after a
+    // barrier, so that no wave still reads the previous tiles).
+    auto product = [&](const TpBlk& X, const TpBlk& Y, f32x16& acc) {
+        float va[kTpPer], vb[kTpPer];
+        load(X, va);
+        load(Y, vb);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kTpPer; ++q) {
+            const int e = tid + 256 * q, x = e % kNB, y = e / kNB; // X: (row x, k y); Y: (k x, col y)
+            As[y][x] = va[q];
+            Bs[x][y] = vb[q];
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < kNB; kk += 2) {
+            const int kr = kk + (lane >> 5);
+            acc = icl::mfma_32x32x2_f32(Bs[kr][wc + (lane & 31)], As[kr][wr + (lane & 31)], acc);
+        }
+    };
+    auto row_of = [&]() { return wr + (lane & 31); };
+    auto col_of = [&](int r) { return wc + icl::mfma32_row(lane, r); };
+    auto store_w = [&](int bi, int bj, const f32x16& acc) { // W_(bi, bj) = -acc
+#pragma unroll
+        for (int r = 0; r < 16; ++r) W[w_at(kNB * bi + row_of(), kNB * bj + col_of(r))] = -acc[r];
+    };
+    auto copy_w = [&](bool inv, int bi, int bj) { // W_(bi, bj) = iD_bi (inv) or 0
+        float v[kTpPer];
+        if (inv) load({TpBlk::INVD, bi, bi}, v);
+        else
+            for (int q = 0; q < kTpPer; ++q) v[q] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kTpPer; ++q) {
+            const int e = tid + 256 * q;
+            W[w_at(kNB * bi + e % kNB, kNB * bj + e / kNB)] = v[q];
+        }
+    };
+
+    if (level == 0) {
+        const int h = blockIdx.y, da = 2 * h, dd = da + 1;
+        copy_w(true, da, da);
+        copy_w(true, dd, dd);
+        copy_w(false, da, dd);
+        if (h == 0)
+            for (int i = 0; i < 2; ++i)
+                for (int j = 2; j < 4; ++j) copy_w(false, i, j);
+        f32x16 s = f32x16{};
+        product({TpBlk::L, dd, da}, {TpBlk::INVD, da, da}, s); // S = L_da iD_a
+        // W_da = -iD_d S: S becomes the B tile, iD_d the A tile.
+        float vd[kTpPer];
+        load({TpBlk::INVD, dd, dd}, vd);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Bs[row_of()][col_of(r)] = s[r];
+#pragma unroll
+        for (int q = 0; q < kTpPer; ++q) {
+            const int e = tid + 256 * q;
+            As[e / kNB][e % kNB] = vd[q];
+        }
+        __syncthreads();
+        f32x16 m = f32x16{};
+#pragma unroll 8
+        for (int kk = 0; kk < kNB; kk += 2) {
+            const int kr = kk + (lane >> 5);
+            m = icl::mfma_32x32x2_f32(Bs[kr][wc + (lane & 31)], As[kr][wr + (lane & 31)], m);
+        }
+        store_w(dd, da, m);
+        return;
+    }
+    const int q = blockIdx.y, r = 2 + (q >> 1), c = q & 1;
+    f32x16 acc = f32x16{};
+    if (level == 1) { // T_rc = sum_{k = c..1} L_rk W_kc
+        for (int k = c; k < 2; ++k) product({TpBlk::L, r, k}, {TpBlk::WBLK, k, c}, acc);
+        float* t = t_blk(r, c);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) t[(long)col_of(e) * kNB + row_of()] = acc[e];
+        return;
+    }
+    for (int k = 2; k <= r; ++k) product({TpBlk::WBLK, r, k}, {TpBlk::TBLK, k, c}, acc);
+    store_w(r, c, acc);
+}
+
+} // namespace
+
+extern "C" hipError_t icl_trsm_invert_panels(const icl_trsm_panel_inv_args* a, hipStream_t stream)
+{
+    if (a->npanels <= 0) return hipSuccess;
+    for (int level = 0; level < 3; ++level) {
+        hipLaunchKernelGGL(trsm_invert_panels_kernel, dim3((unsigned)a->npanels, level == 0 ? 2 : 4), dim3(256), 0,
+                           stream, *a, level);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 extern "C" hipError_t icl_trsm_panel_batch(const icl_trsm_blocked_args* a, int npanels, hipStream_t stream)
 {
     if (a->nlines <= 0 || npanels <= 0) return hipSuccess;
