@@ -245,6 +245,38 @@ def test_sgemm_config2_8192(dev_handle, big, mode, ta, tb):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_sgemm_config2_8192_full_matrix(dev_handle, big, mode):
+    """C2 NN at 8192^3, every one of the 8192^2 elements against a float64
+    product formed on the device (the sampled oracle check above pins the float64
+    product's convention on the same data): an error confined to a few tiles (a
+    tile raster, XCD remap or partial-product launch mistake) cannot pass."""
+    import torch
+    A, B = big
+    n = 8192
+    h = dev_handle
+    h.set_math_mode(mode)
+    dA, dB = _to_dev(A), _to_dev(B)  # storage (cols, ld): the transposes of the column-major matrices
+    dC = torch.empty((n, n), device="cuda", dtype=torch.float32)
+    ib.check(ib.iclblasSgemm(h.value, 0, 0, n, n, n, 1.0, dA.data_ptr(), n, dB.data_ptr(), n, 0.0, dC.data_ptr(), n))
+    torch.cuda.synchronize()
+    h.set_math_mode(ib.MATH_FP32)
+    # storage(C) = C^T = B^T A^T
+    At, Bt = dA.double(), dB.double()
+    ref = Bt @ At
+    err = (dC.double() - ref).abs()
+    del ref
+    absab = Bt.abs() @ At.abs()
+    ratio = float((err / (n * 2.0 ** -24 * absab).clamp_min(1e-300)).max())
+    assert ratio <= 4.0, ratio
+    # the sampled oracle elements agree with the device product used here
+    rows = np.array([0, 1, 4095, 4096, 8191])
+    cols = np.array([8191, 4096, 1, 4095, 0])
+    c64, _ = oracle.dgemm_sample(0, 0, n, 1.0, A, B, rows, cols)
+    dev64 = (Bt @ At)[torch.from_numpy(cols).cuda(), torch.from_numpy(rows).cuda()].cpu().numpy()
+    assert np.allclose(dev64, c64, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_sgemm_config2_nonsquare_padded(dev_handle, mode):
     import torch
     m, n, k, ld = 4096, 8192, 2048, 4160
