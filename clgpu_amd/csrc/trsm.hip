@@ -88,70 +88,106 @@ __global__ void __launch_bounds__(64) trsm_diag_kernel(icl_trsm_diag_args a)
 }
 
 // Inverts every 64 x 64 diagonal block of the canonical lower-triangular system
-// (workgroup b handles rows/columns [64 b, 64 b + nb)): lane j solves
-// L_bb y = e_j, and column j of inv(L_bb) is written to
-// invD[b * 4096 + j * 64 + i] (column-major, rows and columns beyond nb zero).
-// The rows of y are split over four waves (wave w owns rows 16 w .. 16 w + 15);
-// per pivot the owning wave divides and publishes y_i through LDS, then every
-// wave updates its later rows -- the recurrence of solve_in_registers, the same
-// operations in the same order, a quarter of them per wave.
-constexpr int kInvRows = kNB / 4;
+// (workgroup b handles rows/columns [64 b, 64 b + nb)) and writes inv(L_bb) to
+// invD[b * 4096 + j * 64 + i] (column-major, rows and columns beyond nb zero),
+// by recursive doubling: each wave inverts one 16 x 16 diagonal sub-block by
+// substitution (lane j forms column j, the sub-block's entries read from LDS,
+// no barrier per pivot), then the 32 x 32 halves and the whole block follow from
+// inv([A 0; C D]) = [inv(A) 0; -inv(D) C inv(A), inv(D)], each a pair of small
+// products spread over the workgroup. Rows beyond nb are an identity block, so
+// the padded inverse is exact there and those entries are written as zeros.
+constexpr int kIvLd = kNB + 1;
 __global__ void __launch_bounds__(256) trsm_invert_kernel(icl_trsm_blocked_args a)
 {
-    __shared__ DiagBlock s;
-    __shared__ float piv[2][kNB];
+    __shared__ float Ls[kNB][kIvLd];  // L(i, k), i >= k; identity beyond nb
+    __shared__ float Is[kNB][kIvLd];  // the inverse, built in place
+    __shared__ float Ts[kNB / 2][kIvLd]; // products' intermediate
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = blockIdx.x;
     const int r0 = b * kNB;
     const int nb = min(kNB, a.nsys - r0);
     const float* L = a.L + (long)r0 * (a.l_si + a.l_sk);
     {
-        // Lt[i][k] = L(k, i) for k > i, rows i of this wave, all loads in flight together
-        float v[kInvRows];
+        // 16 entries per thread, all loads in flight together: entry e = tid + 256 q
+        // is (i, k) = (e / 64, e % 64), so a wave reads one row.
+        // Each entry, the diagonal included, has exactly one writer here.
+        float v[16];
 #pragma unroll
-        for (int ii = 0; ii < kInvRows; ++ii) {
-            const int i = kInvRows * wave + ii;
-            const bool used = i < nb && lane < nb && lane > i;
-            v[ii] = L[used ? (long)lane * a.l_si + (long)i * a.l_sk : 0];
+        for (int q = 0; q < 16; ++q) {
+            const int e = tid + 256 * q, i = e / kNB, k = e % kNB;
+            const bool used = i < nb && (k < i || (k == i && !a.unit));
+            v[q] = L[used ? (long)i * a.l_si + (long)k * a.l_sk : 0];
         }
 #pragma unroll
-        for (int ii = 0; ii < kInvRows; ++ii) {
-            const int i = kInvRows * wave + ii;
-            s.Lt[i][lane] = (i < nb && lane < nb && lane > i) ? v[ii] : 0.0f;
+        for (int q = 0; q < 16; ++q) {
+            const int e = tid + 256 * q, i = e / kNB, k = e % kNB;
+            const bool used = i < nb && (k < i || (k == i && !a.unit));
+            Ls[i][k] = used ? v[q] : (k == i ? 1.0f : 0.0f);
+            Is[i][k] = 0.0f;
         }
-        if (wave == 0 && lane < nb) s.diag[lane] = a.unit ? 1.0f : L[(long)lane * (a.l_si + a.l_sk)];
     }
     __syncthreads();
-    const int wv = __builtin_amdgcn_readfirstlane(wave); // the same in every lane: scalar branches below
-    const int row0 = kInvRows * wv;
-    float x[kInvRows];
+    // 16 x 16 diagonal sub-block w: lane j < 16 forms column j of its inverse.
+    {
+        const int o = 16 * wave, j = lane;
+        if (j < 16) {
+            float x[16];
 #pragma unroll
-    for (int r = 0; r < kInvRows; ++r) x[r] = (row0 + r == lane && lane < nb) ? 1.0f : 0.0f;
-    for (int ib = 0; ib < kNB; ib += kInvRows) {
+            for (int i = 0; i < 16; ++i) {
+                float acc = i == j ? 1.0f : 0.0f;
 #pragma unroll
-        for (int ii = 0; ii < kInvRows; ++ii) {
-            const int i = ib + ii;
-            if (i >= nb) break;
-            if (wv == ib / kInvRows) {
-                float xi = x[ii];
-                if (!a.unit) xi = xi / s.diag[i];
-                x[ii] = xi;
-                piv[i & 1][lane] = xi;
+                for (int k = 0; k < i; ++k) acc -= Ls[o + i][o + k] * x[k];
+                x[i] = acc / Ls[o + i][o + i];
             }
-            __syncthreads(); // piv[i & 1] is rewritten two pivots later, after the next barrier
-            const float xi = piv[i & 1][lane];
-            if (wv > ib / kInvRows) {
 #pragma unroll
-                for (int r = 0; r < kInvRows; ++r) x[r] -= s.Lt[i][row0 + r] * xi;
-            } else if (wv == ib / kInvRows) {
-#pragma unroll
-                for (int r = ii + 1; r < kInvRows; ++r) x[r] -= s.Lt[i][row0 + r] * xi;
-            }
+            for (int i = 0; i < 16; ++i) Is[o + i][o + j] = x[i];
         }
     }
-    float* col = a.invD + (long)b * kNB * kNB + (long)lane * kNB + row0;
+    __syncthreads();
+    // Halves: for the 32 x 32 blocks {0, 1} and {2, 3} (offsets 0 and 32), then the
+    // whole block (offset 0, size 32): inv(D) C inv(A) with A at [o, o + h), D at
+    // [o + h, o + 2 h), C = L[o + h .., o ..]; T = C inv(A), then -inv(D) T.
 #pragma unroll
-    for (int r = 0; r < kInvRows; ++r) col[r] = (lane < nb && row0 + r < nb) ? x[r] : 0.0f;
+    for (int level = 0; level < 2; ++level) {
+        const int h = level == 0 ? 16 : 32;          // size of A, C, D
+        const int nblk = level == 0 ? 2 : 1;         // independent blocks
+        const int per = (nblk * h * h + 255) / 256;  // elements per thread
+#pragma unroll
+        for (int step = 0; step < 2; ++step) {
+            float out[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q >= per) break;
+                const int e = tid + 256 * q;
+                const int blk = e / (h * h), rc = e % (h * h), r = rc / h, c = rc % h;
+                const int o = blk * 2 * h;
+                float acc = 0.0f;
+                if (step == 0) { // T(r, c) = sum_k C(r, k) inv(A)(k, c), k >= c
+                    for (int k = c; k < h; ++k) acc += Ls[o + h + r][o + k] * Is[o + k][o + c];
+                } else { // -sum_k inv(D)(r, k) T(k, c), k <= r
+                    for (int k = 0; k <= r; ++k) acc -= Is[o + h + r][o + h + k] * Ts[blk * h + k][c];
+                }
+                out[q] = acc;
+            }
+            __syncthreads(); // every read of Ts / Is of this step is done
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q >= per) break;
+                const int e = tid + 256 * q;
+                const int blk = e / (h * h), rc = e % (h * h), r = rc / h, c = rc % h;
+                const int o = blk * 2 * h;
+                if (step == 0) Ts[blk * h + r][c] = out[q];
+                else Is[o + h + r][o + c] = out[q];
+            }
+            __syncthreads();
+        }
+    }
+    float* dst = a.invD + (long)b * kNB * kNB;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int e = tid + 256 * q, j = e / kNB, i = e % kNB; // column j, row i: consecutive threads along i
+        dst[e] = (i < nb && j < nb && i >= j) ? Is[i][j] : 0.0f;
+    }
     if (a.sync && b == 0) { // the dataflow solve that follows on this stream starts from zeroed flags
         const int n = ((a.nsys + kNB - 1) / kNB) * ((a.nlines + kNB - 1) / kNB) + 2;
         for (int i = tid; i < n; i += 256) a.sync[i] = 0;
