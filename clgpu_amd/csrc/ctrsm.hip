@@ -100,88 +100,126 @@ extern "C" hipError_t icl_ctrsm_diag(const icl_ctrsm_diag_args* a, hipStream_t s
 namespace {
 
 // Inverts every 64 x 64 diagonal block of the canonical system (workgroup b
-// handles unknowns [64 b, 64 b + nb)): lane j solves L_bb y = e_j with the
-// reference's complex arithmetic and column j of inv(L_bb) is written to
-// invD[b * 4096 + j * 64 + i] (interleaved complex, rows and columns beyond
-// nb zero). L(i, k) = L[i * l_si + k * l_sk], conjugated when conj. The rows of
-// y are split over four waves (wave w owns rows 16 w .. 16 w + 15); per pivot
-// the owning wave divides and publishes y_i through LDS, then every wave
-// updates its later rows -- the same operations in the same order as a
-// one-wave substitution, a quarter of them per wave.
-constexpr int kInvRows = kNB / 4;
+// handles unknowns [64 b, 64 b + nb)) and writes inv(L_bb) to
+// invD[b * 4096 + j * 64 + i] (interleaved complex, column-major, rows and
+// columns beyond nb zero). L(i, k) = L[i * l_si + k * l_sk], conjugated when
+// conj. Recursive doubling: each wave inverts one 16 x 16 diagonal sub-block by
+// substitution with the reference's complex multiply and divide (lane j forms
+// column j, the sub-block read from LDS, no barrier per pivot), then the 32 x 32
+// halves and the whole block follow from
+// inv([A 0; C D]) = [inv(A) 0; -inv(D) C inv(A), inv(D)], each a pair of small
+// complex products spread over the workgroup. Rows beyond nb are an identity
+// block, so the padded inverse is exact there and those entries are written as
+// zeros. The three LDS arrays (L, the inverse, the products' intermediate) take
+// kCivLds bytes of dynamic LDS.
+constexpr int kCivLd = kNB + 1;
+constexpr size_t kCivLds = sizeof(float2) * (2 * kNB + kNB / 2) * kCivLd;
 __global__ void __launch_bounds__(256) ctrsm_invert_kernel(icl_ctrsm_blocked_args a)
 {
-    __shared__ CDiagBlock s;
-    __shared__ float2 piv[2][kNB];
+    extern __shared__ float sm[];
+    float2 (*Ls)[kCivLd] = reinterpret_cast<float2 (*)[kCivLd]>(sm); // L(i, k), i >= k; identity beyond nb
+    float2 (*Is)[kCivLd] = Ls + kNB;                                  // the inverse, built in place
+    float2 (*Ts)[kCivLd] = Is + kNB;                                  // products' intermediate
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = blockIdx.x;
     const int r0 = b * kNB;
     const int nb = min(kNB, a.nsys - r0);
     const float2* L = reinterpret_cast<const float2*>(a.L) + (long)r0 * (a.l_si + a.l_sk);
     const float sg = a.conj ? -1.0f : 1.0f;
-    // All loads first (an entry outside the strict lower triangle reads the
-    // block's first diagonal element instead and is replaced by 0), so that they
-    // are in flight together.
-    float2 lv[kInvRows];
+    {
+        // 16 entries per thread, all loads in flight together: entry e = tid + 256 q
+        // is (i, k) = (e / 64, e % 64); an entry outside the used triangle reads the
+        // block's first diagonal element instead and is replaced. Each entry, the
+        // diagonal included, has exactly one writer here.
+        float2 v[16];
 #pragma unroll
-    for (int ii = 0; ii < kInvRows; ++ii) {
-        const int i = kInvRows * wave + ii;
-        const bool used = i < nb && lane < nb && lane > i;
-        lv[ii] = L[used ? (long)lane * a.l_si + (long)i * a.l_sk : 0];
-    }
-#pragma unroll
-    for (int ii = 0; ii < kInvRows; ++ii) {
-        const int i = kInvRows * wave + ii;
-        const bool used = i < nb && lane < nb && lane > i;
-        s.Lt[i][lane] = used ? make_float2(lv[ii].x, sg * lv[ii].y) : make_float2(0.0f, 0.0f);
-    }
-    if (wave == 0 && lane < nb) {
-        float2 d = make_float2(1.0f, 0.0f);
-        if (!a.unit) {
-            d = L[(long)lane * (a.l_si + a.l_sk)];
-            d.y *= sg;
+        for (int q = 0; q < 16; ++q) {
+            const int e = tid + 256 * q, i = e / kNB, k = e % kNB;
+            const bool used = i < nb && (k < i || (k == i && !a.unit));
+            v[q] = L[used ? (long)i * a.l_si + (long)k * a.l_sk : 0];
         }
-        s.diag[lane] = d;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = tid + 256 * q, i = e / kNB, k = e % kNB;
+            const bool used = i < nb && (k < i || (k == i && !a.unit));
+            Ls[i][k] = used ? make_float2(v[q].x, sg * v[q].y) : make_float2(k == i ? 1.0f : 0.0f, 0.0f);
+            Is[i][k] = make_float2(0.0f, 0.0f);
+        }
     }
     __syncthreads();
-    const int wv = __builtin_amdgcn_readfirstlane(wave); // the same in every lane: scalar branches below
-    const int row0 = kInvRows * wv;
-    float2 x[kInvRows];
+    // 16 x 16 diagonal sub-block w: lane j < 16 forms column j of its inverse.
+    {
+        const int o = 16 * wave, j = lane;
+        if (j < 16) {
+            float2 x[16];
 #pragma unroll
-    for (int r = 0; r < kInvRows; ++r) x[r] = make_float2(row0 + r == lane && lane < nb ? 1.0f : 0.0f, 0.0f);
-    for (int ib = 0; ib < kNB; ib += kInvRows) {
+            for (int i = 0; i < 16; ++i) {
+                float2 acc = make_float2(i == j ? 1.0f : 0.0f, 0.0f);
 #pragma unroll
-        for (int ii = 0; ii < kInvRows; ++ii) {
-            const int i = ib + ii;
-            if (i >= nb) break;
-            if (wv == ib / kInvRows) {
-                float2 xi = x[ii];
-                if (!a.unit) xi = c_div(xi, s.diag[i]);
-                x[ii] = xi;
-                piv[i & 1][lane] = xi;
-            }
-            __syncthreads(); // piv[i & 1] is rewritten two pivots later, after the next barrier
-            const float2 xi = piv[i & 1][lane];
-            if (wv > ib / kInvRows) {
-#pragma unroll
-                for (int r = 0; r < kInvRows; ++r) {
-                    const float2 t = c_mul(s.Lt[i][row0 + r], xi);
-                    x[r].x -= t.x;
-                    x[r].y -= t.y;
+                for (int k = 0; k < i; ++k) {
+                    const float2 t = c_mul(Ls[o + i][o + k], x[k]);
+                    acc.x -= t.x;
+                    acc.y -= t.y;
                 }
-            } else if (wv == ib / kInvRows) {
-#pragma unroll
-                for (int r = ii + 1; r < kInvRows; ++r) {
-                    const float2 t = c_mul(s.Lt[i][row0 + r], xi);
-                    x[r].x -= t.x;
-                    x[r].y -= t.y;
-                }
+                x[i] = a.unit ? acc : c_div(acc, Ls[o + i][o + i]);
             }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Is[o + i][o + j] = x[i];
         }
     }
-    float2* col = reinterpret_cast<float2*>(a.invD) + (long)b * kNB * kNB + (long)lane * kNB + row0;
+    __syncthreads();
+    // Halves: for the 32 x 32 blocks {0, 1} and {2, 3} (offsets 0 and 32), then the
+    // whole block (offset 0, size 32): inv(D) C inv(A) with A at [o, o + h), D at
+    // [o + h, o + 2 h), C = L[o + h .., o ..]; T = C inv(A), then -inv(D) T.
 #pragma unroll
-    for (int r = 0; r < kInvRows; ++r) col[r] = (lane < nb && row0 + r < nb) ? x[r] : make_float2(0.0f, 0.0f);
+    for (int level = 0; level < 2; ++level) {
+        const int h = level == 0 ? 16 : 32;          // size of A, C, D
+        const int nblk = level == 0 ? 2 : 1;         // independent blocks
+        const int per = (nblk * h * h + 255) / 256;  // elements per thread
+#pragma unroll
+        for (int step = 0; step < 2; ++step) {
+            float2 out[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q >= per) break;
+                const int e = tid + 256 * q;
+                const int blk = e / (h * h), rc = e % (h * h), r = rc / h, c = rc % h;
+                const int o = blk * 2 * h;
+                float2 acc = make_float2(0.0f, 0.0f);
+                if (step == 0) { // T(r, c) = sum_k C(r, k) inv(A)(k, c), k >= c
+                    for (int k = c; k < h; ++k) {
+                        const float2 t = c_mul(Ls[o + h + r][o + k], Is[o + k][o + c]);
+                        acc.x += t.x;
+                        acc.y += t.y;
+                    }
+                } else { // -sum_k inv(D)(r, k) T(k, c), k <= r
+                    for (int k = 0; k <= r; ++k) {
+                        const float2 t = c_mul(Is[o + h + r][o + h + k], Ts[blk * h + k][c]);
+                        acc.x -= t.x;
+                        acc.y -= t.y;
+                    }
+                }
+                out[q] = acc;
+            }
+            __syncthreads(); // every read of Ts / Is of this step is done
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q >= per) break;
+                const int e = tid + 256 * q;
+                const int blk = e / (h * h), rc = e % (h * h), r = rc / h, c = rc % h;
+                const int o = blk * 2 * h;
+                if (step == 0) Ts[blk * h + r][c] = out[q];
+                else Is[o + h + r][o + c] = out[q];
+            }
+            __syncthreads();
+        }
+    }
+    float2* dst = reinterpret_cast<float2*>(a.invD) + (long)b * kNB * kNB;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int e = tid + 256 * q, j = e / kNB, i = e % kNB; // column j, row i: consecutive threads along i
+        dst[e] = (i < nb && j < nb && i >= j) ? Is[i][j] : make_float2(0.0f, 0.0f);
+    }
 }
 
 // X_b = s_post * inv(L_bb) B_b for one diagonal block and all lines, in place:
@@ -494,8 +532,11 @@ extern "C" hipError_t icl_ctrsm_invert_panels(const icl_ctrsm_panel_inv_args* a,
 extern "C" hipError_t icl_ctrsm_invert_blocks(const icl_ctrsm_blocked_args* a, hipStream_t stream)
 {
     if (a->nsys <= 0) return hipSuccess;
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(ctrsm_invert_kernel),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCivLds);
+    if (attr != hipSuccess) return attr;
     const unsigned blocks = (unsigned)((a->nsys + kNB - 1) / kNB);
-    hipLaunchKernelGGL(ctrsm_invert_kernel, dim3(blocks), dim3(256), 0, stream, *a);
+    hipLaunchKernelGGL(ctrsm_invert_kernel, dim3(blocks), dim3(256), kCivLds, stream, *a);
     return hipGetLastError();
 }
 
