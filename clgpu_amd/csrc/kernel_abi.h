@@ -266,7 +266,6 @@ typedef struct icl_trsm_blocked_args {
     int unit;
     int* sync;       /* dataflow solve only: icl_trsm_dataflow_sync_ints(...) ints, zeroed by invert_blocks */
     int* error_word; /* dataflow solve only: set to 1 (never cleared by the kernels) when a wait is aborted */
-    long x_batch;    /* icl_trsm_panel_batch only: X offset (elements) between consecutive panels */
     int wait_polls;  /* dataflow solve only: polls of a flag per wait before the wait aborts; 0 = the
                       * default bound (65536), < 0 = abort the first wait without polling */
 } icl_trsm_blocked_args;
@@ -294,13 +293,6 @@ int icl_trsm_dataflow_sync_ints(int nsys, int nlines);
  * [ps, pe) through the X view. B rows of the panel are read, not written; rows
  * outside the panel are not touched (the caller updates them, e.g. with a GEMM). */
 hipError_t icl_trsm_panel(const icl_trsm_blocked_args* a, int ps, int pe, hipStream_t stream);
-/* The panel solve for panels 0 .. npanels - 1 (rows [256 p, 256 p + 256)) in
- * one launch, panel p writing its rows of X at a->X + p * a->x_batch, with row
- * index relative to the panel's first row (x_batch > 0). */
-hipError_t icl_trsm_panel_batch(const icl_trsm_blocked_args* a, int npanels, hipStream_t stream);
-/* E (nsys x width, column-major, ld nsys) = a vertical stack of width x width
- * identity blocks: E(r, l) = 1 if r % width == l, else 0. */
-hipError_t icl_identity_stack(float* E, int nsys, int width, hipStream_t stream);
 
 /* Inverses W_p of the npanels full 256 x 256 diagonal panels of the canonical
  * lower-triangular L (L(i, k) = L[i * l_si + k * l_sk]) from its inverted 64 x 64

@@ -1027,10 +1027,6 @@ constexpr size_t kPanelLds = sizeof(float) * ((size_t)kPanelRows * kPanelLd + (s
 template <bool LINES_CONTIG, bool L_ROWS_CONTIG>
 __global__ void __launch_bounds__(256) trsm_panel_kernel(icl_trsm_blocked_args a, int ps, int pe)
 {
-    if (a.x_batch) { // batched: panel blockIdx.y, X rows relative to the panel
-        ps = blockIdx.y * kPanelRows;
-        pe = min(a.nsys, ps + kPanelRows);
-    }
     extern __shared__ __attribute__((aligned(16))) float sm[];
     float* P = sm;                      // P[row][line]: panel rows of B, then of X
     float* T = P + kPanelRows * kPanelLd; // T[k][r]: operand tile of the current product
@@ -1115,8 +1111,7 @@ __global__ void __launch_bounds__(256) trsm_panel_kernel(icl_trsm_blocked_args a
     // X rows [ps, pe) through the X view, lanes along its contiguous dimension:
     // 64 elements per thread, addresses from one base plus uniform strides.
     {
-        float* dst = a.x_batch ? a.X + (long)blockIdx.y * a.x_batch + (long)c0 * a.xv_sj
-                               : a.X + (long)ps * a.xv_si + (long)c0 * a.xv_sj;
+        float* dst = a.X + (long)ps * a.xv_si + (long)c0 * a.xv_sj;
         if (LINES_CONTIG) { // thread: line tid % 64, rows tid / 64 + 4 j
             const int l = tid % kStepCols, r0t = tid / kStepCols;
             float* d = dst + (long)r0t * a.xv_si + (long)l * a.xv_sj;
@@ -1268,7 +1263,7 @@ namespace {
 
 using PanelKern = void (*)(icl_trsm_blocked_args, int, int);
 
-hipError_t launch_panel(const icl_trsm_blocked_args* a, int ps, int pe, int npanels, hipStream_t stream)
+hipError_t launch_panel(const icl_trsm_blocked_args* a, int ps, int pe, hipStream_t stream)
 {
     using Kern = PanelKern;
     static const Kern kernels[4] = {trsm_panel_kernel<false, false>, trsm_panel_kernel<false, true>,
@@ -1285,18 +1280,9 @@ hipError_t launch_panel(const icl_trsm_blocked_args* a, int ps, int pe, int npan
     const bool lines_contig = a->x_sj == 1 || a->x_sj == -1;
     const bool l_rows_contig = a->l_si == 1 || a->l_si == -1;
     const unsigned blocks = (unsigned)((a->nlines + kStepCols - 1) / kStepCols);
-    hipLaunchKernelGGL(kernels[2 * lines_contig + l_rows_contig], dim3(blocks, npanels), dim3(256), kPanelLds, stream,
-                       *a, ps, pe);
+    hipLaunchKernelGGL(kernels[2 * lines_contig + l_rows_contig], dim3(blocks), dim3(256), kPanelLds, stream, *a, ps,
+                       pe);
     return hipGetLastError();
-}
-
-__global__ void identity_stack_kernel(float* E, int nsys, int width)
-{
-    const long total = (long)nsys * width;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-        const long l = e / nsys, r = e - l * nsys;
-        E[e] = (r % width == l) ? 1.0f : 0.0f;
-    }
 }
 
 } // namespace
@@ -1304,8 +1290,8 @@ __global__ void identity_stack_kernel(float* E, int nsys, int width)
 extern "C" hipError_t icl_trsm_panel(const icl_trsm_blocked_args* a, int ps, int pe, hipStream_t stream)
 {
     if (a->nlines <= 0 || pe <= ps) return hipSuccess;
-    if (ps % kNB != 0 || pe - ps > kPanelRows || pe > a->nsys || a->x_batch != 0) return hipErrorInvalidValue;
-    return launch_panel(a, ps, pe, 1, stream);
+    if (ps % kNB != 0 || pe - ps > kPanelRows || pe > a->nsys) return hipErrorInvalidValue;
+    return launch_panel(a, ps, pe, stream);
 }
 
 namespace {
@@ -1447,21 +1433,6 @@ extern "C" hipError_t icl_trsm_invert_panels(const icl_trsm_panel_inv_args* a, h
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
-}
-
-extern "C" hipError_t icl_trsm_panel_batch(const icl_trsm_blocked_args* a, int npanels, hipStream_t stream)
-{
-    if (a->nlines <= 0 || npanels <= 0) return hipSuccess;
-    if (a->x_batch <= 0 || (long)npanels * kPanelRows > a->nsys + kPanelRows - 1) return hipErrorInvalidValue;
-    return launch_panel(a, 0, 0, npanels, stream);
-}
-
-extern "C" hipError_t icl_identity_stack(float* E, int nsys, int width, hipStream_t stream)
-{
-    if (nsys <= 0 || width <= 0) return hipSuccess;
-    hipLaunchKernelGGL(identity_stack_kernel, dim3(elementwise_grid((long)nsys * width)), dim3(256), 0, stream, E, nsys,
-                       width);
-    return hipGetLastError();
 }
 
 extern "C" hipError_t icl_trsm_scatter(const icl_trsm_blocked_args* a, float alpha, hipStream_t stream)
