@@ -8,10 +8,10 @@
 //
 // op N: a workgroup owns 64 * VW rows (VW = 4 floats or 2 complex per lane, one
 //       16-byte load) and a range of columns; its four waves take interleaved
-//       columns, four in flight per lane, and are summed through LDS.
+//       columns, eight in flight per lane, and are summed through LDS.
 // op T / C: a wave owns two columns and walks their rows (or a split's share of
-//       them) 64 * VW at a time, reusing each x load for both columns, then
-//       reduces across its lanes.
+//       them) two steps of 64 * VW at a time, reusing each x load for both
+//       columns, then reduces across its lanes.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -23,7 +23,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kColsPerWave = 2;  // op T / C
-constexpr int kUnroll = 4;       // independent columns (op N) or row steps (op T) in flight
+constexpr int kUnroll = 8;       // independent columns per lane in flight (op N)
 
 template <typename T>
 struct Num;
@@ -181,17 +181,41 @@ __global__ void __launch_bounds__(kThreads) gemv_t_kernel(icl_gemv_args a)
     const T* c1p = A + (long)(two ? j0 + 1 : j0) * a.lda;
 
     T acc0 = Num<T>::zero(), acc1 = Num<T>::zero();
-    for (int i = r0 + lane * VW; i < r1; i += STEP) {
-        const int count = min(VW, r1 - i);
-        T v0[VW], v1[VW], xv[VW];
-        load_run<T, VEC>(c0p + i, count, v0);
-        load_run<T, VEC>(c1p + i, count, v1);
+    auto load_x = [&](int i, int count, T (&xv)[VW]) {
         if constexpr (XVEC) {
             load_run<T, true>(x + i, count, xv);
         } else {
 #pragma unroll
             for (int e = 0; e < VW; ++e) xv[e] = e < count ? x[(long)(i + e) * a.incx] : Num<T>::zero();
         }
+    };
+    int i = r0 + lane * VW;
+    // two full row steps per iteration, all their loads in flight together (the
+    // same products in the same order as one step at a time)
+    for (; i + STEP + VW <= r1; i += 2 * STEP) {
+        T v0[2][VW], v1[2][VW], xv[2][VW];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            load_run<T, VEC>(c0p + i + u * STEP, VW, v0[u]);
+            load_run<T, VEC>(c1p + i + u * STEP, VW, v1[u]);
+            load_x(i + u * STEP, VW, xv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                const T p0 = CONJ ? Num<T>::conj(v0[u][e]) : v0[u][e];
+                const T p1 = CONJ ? Num<T>::conj(v1[u][e]) : v1[u][e];
+                acc0 = Num<T>::fma(p0, xv[u][e], acc0);
+                acc1 = Num<T>::fma(p1, xv[u][e], acc1);
+            }
+    }
+    for (; i < r1; i += STEP) {
+        const int count = min(VW, r1 - i);
+        T v0[VW], v1[VW], xv[VW];
+        load_run<T, VEC>(c0p + i, count, v0);
+        load_run<T, VEC>(c1p + i, count, v1);
+        load_x(i, count, xv);
 #pragma unroll
         for (int e = 0; e < VW; ++e) {
             const T p0 = CONJ ? Num<T>::conj(v0[e]) : v0[e];
