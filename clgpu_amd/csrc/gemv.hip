@@ -8,7 +8,7 @@
 //
 // op N: a workgroup owns 64 * VW rows (VW = 4 floats or 2 complex per lane, one
 //       16-byte load) and a range of columns; its four waves take interleaved
-//       columns, eight in flight per lane, and are summed through LDS.
+//       columns, sixteen in flight per lane, and are summed through LDS.
 // op T / C: a wave owns two columns and walks their rows (or a split's share of
 //       them) two steps of 64 * VW at a time, reusing each x load for both
 //       columns, then reduces across its lanes.
@@ -23,7 +23,8 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kColsPerWave = 2;  // op T / C
-constexpr int kUnroll = 8;       // independent columns per lane in flight (op N)
+constexpr int kUnroll = 16;      // independent columns per lane in flight (op N)
+constexpr int kStepsT = 2;       // row steps per iteration (op T / C)
 
 template <typename T>
 struct Num;
@@ -192,16 +193,16 @@ __global__ void __launch_bounds__(kThreads) gemv_t_kernel(icl_gemv_args a)
     int i = r0 + lane * VW;
     // two full row steps per iteration, all their loads in flight together (the
     // same products in the same order as one step at a time)
-    for (; i + STEP + VW <= r1; i += 2 * STEP) {
-        T v0[2][VW], v1[2][VW], xv[2][VW];
+    for (; i + (kStepsT - 1) * STEP + VW <= r1; i += kStepsT * STEP) {
+        T v0[kStepsT][VW], v1[kStepsT][VW], xv[kStepsT][VW];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < kStepsT; ++u) {
             load_run<T, VEC>(c0p + i + u * STEP, VW, v0[u]);
             load_run<T, VEC>(c1p + i + u * STEP, VW, v1[u]);
             load_x(i + u * STEP, VW, xv[u]);
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < kStepsT; ++u)
 #pragma unroll
             for (int e = 0; e < VW; ++e) {
                 const T p0 = CONJ ? Num<T>::conj(v0[u][e]) : v0[u][e];
