@@ -141,7 +141,10 @@ const icl_gemm_variant* gemm_select(iclblasContext* h, const GemmProblem& p)
 // 3 k 2^-24 |a||b| once k >= 256. Smaller K runs on FP32 MFMA instead.
 constexpr int kSplitMinK = 256;
 
-static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, const icl_gemm_args& g, bool record)
+// `stream`: the stream to run on (default the handle's; a variant with a
+// workspace always runs on the handle's stream).
+static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, const icl_gemm_args& g, bool record,
+                     hipStream_t stream = nullptr)
 {
     if (math == ICL_MATH_BF16X3 && g.K < kSplitMinK) math = ICL_MATH_FP32;
     GemmProblem p{kind, math, opa, opb, &g};
@@ -164,14 +167,17 @@ static void run_gemm(iclblasContext* h, int kind, int math, int opa, int opb, co
     // and may run half of the prepass on the handle's side stream: offered for
     // products of at least 1024 tiles of that variant, never to a call that must
     // not allocate (the stream and its events are created on first use).
-    if (!v->workspace) hip_check(icl_timer_mark(h->stream), "icl_timer_mark");
+    if (v->workspace && stream && stream != h->stream)
+        throw status_error(ICLBLAS_STATUS_INTERNAL_ERROR, "GEMM with a workspace off the handle's stream");
+    const hipStream_t s = stream ? stream : h->stream;
+    if (!v->workspace) hip_check(icl_timer_mark(s), "icl_timer_mark");
     const long tiles = (long)((g.M + v->bm - 1) / v->bm) * ((g.N + v->bn - 1) / v->bn);
     struct SideGuard {
         explicit SideGuard(const icl_side_lane* l) { t_side_lane = l; }
         ~SideGuard() { t_side_lane = nullptr; }
     } side_guard(v->workspace && ws && !h->ext_ws && tiles >= 1024 && !level3_problem(g) ? h->side.ensure() : nullptr);
-    hip_check(v->launch(&g, opa, opb, edge ? 1 : 0, ws, h->stream), v->name);
-    if (!v->workspace) hip_check(icl_timer_mark(h->stream), "icl_timer_mark");
+    hip_check(v->launch(&g, opa, opb, edge ? 1 : 0, ws, s), v->name);
+    if (!v->workspace) hip_check(icl_timer_mark(s), "icl_timer_mark");
     if (record) h->last_impl = v->name;
 }
 
@@ -278,7 +284,26 @@ void gemm_level3_dev(iclblasContext* h, int kind, int opa, int opb, int m, int n
     g.kband = kband;
     g.cflags = cflags;
     const int math = split_depth(h, k) ? ICL_MATH_BF16X3 : ICL_MATH_FP32;
-    if (band_split_pays(h, kind, math, opa, opb, g)) {
+    const bool split = band_split_pays(h, kind, math, opa, opb, g);
+    if (split && opa == 0) {
+        // The two launches write disjoint elements of C. With op(A) = A the band
+        // runs on the handle's side stream beside the off-band launch, its small
+        // workgroups sharing the CUs with the large ones (forked after the work
+        // already queued, joined before the call returns): SSYRK 8192 op N 4.40 ->
+        // 4.25 ms. With op(A) transposed the pair ran slower side by side (4.35 ->
+        // 4.85 ms), so those launches stay one after the other.
+        const icl_side_lane* side = h->side.ensure();
+        hip_check(hipEventRecord(side->start, h->stream), "hipEventRecord");
+        g.write_tri = write_tri | ICL_TRI_OFF_BAND;
+        run_gemm(h, kind, math, opa, opb, g, true);
+        hip_check(hipStreamWaitEvent(side->stream, side->start, 0), "hipStreamWaitEvent");
+        g.write_tri = write_tri | ICL_TRI_BAND;
+        run_gemm(h, kind, math, opa, opb, g, true, side->stream);
+        hip_check(hipEventRecord(side->done, side->stream), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(h->stream, side->done, 0), "hipStreamWaitEvent");
+        return;
+    }
+    if (split) {
         g.write_tri = write_tri | ICL_TRI_OFF_BAND;
         run_gemm(h, kind, math, opa, opb, g, true);
         g.write_tri = write_tri | ICL_TRI_BAND;
